@@ -1,0 +1,230 @@
+"""Full-grid operator-split step throughput on one MI355X.
+
+A "step" is one pass of the hot path over the whole grid: convection
+(Convection_2D over every (V,K) plane), radial diffusion (Diffusion_1D over
+every (P,V,K) line) and local diffusion (Diffusion_2D over every (P,R) node),
+src/VERB4D_Solver.cpp:851-1216, on the production grid 29x24x101x91
+(R x MLT x E x alpha) with synthetic seeded inputs already resident in HBM.
+LU caching is off (every step factors all 696 band systems), the plain
+FP64 path.
+
+Prints one JSON line. Besides the driver's fields it carries
+  roofline      the dominant kernel's achieved rate (algorithmic work per
+                launch / its average launch time, HIP events on the library
+                stream) against the MI355X peak
+  cpu_baseline  the reference's own CPU code (oracle/_ref, compiled from the
+                PRBEM/VERB-4D sources, OpenMP + OpenBLAS) timed on this
+                host on a bounded sample of the same step
+"""
+import os
+
+os.environ.setdefault("OPENBLAS_NUM_THREADS", "1")  # reference: single-threaded BLAS per call
+
+import argparse  # noqa: E402
+import json  # noqa: E402
+import sys  # noqa: E402
+import time  # noqa: E402
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "verb-4d_amd"))
+
+METRIC = "full-grid steps/s at 29×24×101×91, ≤1e-8 rel err vs FP64; % of roofline"
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def lu_flops(n, kl, ku):
+    """Structural band-LU flops without fill: sum_j (l_j + 2 l_j u_j)."""
+    import numpy as np
+
+    j = np.arange(n, dtype=np.float64)
+    l = np.minimum(kl, n - 1 - j)
+    u = np.minimum(ku, n - 1 - j)
+    return float(np.sum(l + 2 * l * u))
+
+
+def phase_work(w):
+    """Algorithmic work per launch of each kernel phase: (amount, unit, bound)."""
+    n = w.nV * w.nK
+    kl = ku = w.nV + 1
+    nodes = w.nP * w.nR
+    pts = w.points
+    return {
+        # PSD read+write, VP, VR, G_conv, losses, sources: 7 doubles / point
+        "convection": (56.0 * pts, "B", "hbm"),
+        # PSD r/w, D_LL, G_radial, L grid, sources, losses: 7 doubles / point
+        "radial": (56.0 * pts, "B", "hbm"),
+        # 8 inputs + 3 outputs per point of the node slices
+        "local_assemble": (88.0 * pts, "B", "hbm"),
+        "local_factor": (lu_flops(n, kl, ku) * nodes, "flop", "mfma"),
+        # factors read once: (2kl+ku+1) doubles per column, plus rhs r/w
+        "local_solve": ((2 * kl + ku + 1) * 8.0 * n * nodes + 16.0 * pts, "B", "hbm"),
+    }
+
+
+def cpu_baseline(w, psd, threads, target_s):
+    """The reference CPU path on a bounded sample of the step: each operator
+    on a growing prefix of its slices until it has run target_s seconds (or
+    covered everything), then scaled to one full step."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    if not O.ref_available():
+        return None
+    totals = {"conv": w.nV * w.nK, "radial": w.nP * w.nV * w.nK, "local": w.nP * w.nR}
+    start = {"conv": 64, "radial": 4096, "local": 16}
+    est = 0.0
+    parts = {}
+    for op in ("conv", "radial", "local"):
+        k = start[op]
+        while True:
+            k = min(k, totals[op])
+            x = psd.copy()
+            args = {"conv": (k, 0, 0), "radial": (0, k, 0), "local": (0, 0, k)}[op]
+            t0 = time.perf_counter()
+            O.ref_step_sample(w, x, threads, *args)
+            dt = time.perf_counter() - t0
+            if dt >= target_s or k == totals[op]:
+                break
+            k *= 2
+        parts[op] = (k, dt)
+        est += dt * totals[op] / k
+        del x
+    sample = ", ".join("%s %d of %d slices in %.2f s" % (op, parts[op][0], totals[op], parts[op][1])
+                       for op in parts)
+    return {"value": 1.0 / est, "unit": "steps/s", "cores": threads, "kind": "reference",
+            "sample": "reference CPU path (OpenMP schedule(dynamic,1), OpenBLAS dgbtrf/dgbtrs single-threaded "
+                      "per call): " + sample + "; scaled to one full step",
+            "seconds_per_step": est}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=3.0, help="target seconds per operator sample")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cached", action="store_true", help="also time steps that reuse cached LU factors")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only; --gpus %d is not supported (the multi-GPU "
+              "sharding is a design note, not an implemented path)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import verb4d
+    from verb4d import workload as W
+    from verb4d.device import DeviceGrid
+
+    torch.cuda.set_device(0)
+    w = W.config1(seed=1001)  # Kp = 2 regime of the production configuration
+    ctx = verb4d.Context()
+    dg = DeviceGrid(w, ctx=ctx)
+    print("bench: workload ready (%d points, %d nodes)" % (w.points, w.nP * w.nR), file=sys.stderr, flush=True)
+
+    for _ in range(args.warmup):
+        dg.step()
+    dg.check_status()
+    torch.cuda.synchronize()
+    ctx.synchronize()
+
+    ext = torch.cuda.ExternalStream(ctx.stream)
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ctx.set_timing(True)
+    ctx.timings()  # clear
+    torch.cuda.synchronize()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ev0.record(ext)
+    for _ in range(args.steps):
+        dg.step()
+    ev1.record(ext)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    phases = ctx.timings()
+    ctx.set_timing(False)
+    dg.check_status()
+    ms_events = ev0.elapsed_time(ev1)
+    ms_per_step = ms_events / args.steps
+    print("bench: %d steps, %.3f ms/step (events), wall %.3f s" % (args.steps, ms_per_step, wall),
+          file=sys.stderr, flush=True)
+
+    work = phase_work(w)
+    phase_out = {}
+    dom, dom_ms = None, -1.0
+    for p, (ms, cnt) in phases.items():
+        if cnt == 0:
+            continue
+        avg = ms / cnt
+        amount, unit, bound = work[p]
+        rate = amount / (avg * 1e-3)
+        if unit == "flop":
+            ach, peak, u = rate / 1e12, PEAK_FP64_TFLOPS, "TFLOP/s"
+        else:
+            ach, peak, u = rate / 1e9, PEAK_HBM_GBS, "GB/s"
+        phase_out[p] = {"avg_ms": round(avg, 4), "achieved": round(ach, 3), "unit": u, "frac": round(ach / peak, 4)}
+        if avg > dom_ms:
+            dom, dom_ms = p, avg
+    amount, unit, bound = work[dom]
+    d = phase_out[dom]
+    roofline = {"bound": bound, "achieved": d["achieved"],
+                "peak": PEAK_FP64_TFLOPS if unit == "flop" else PEAK_HBM_GBS, "unit": d["unit"],
+                "frac": d["frac"], "traffic": None, "kernel": dom}
+
+    cached = None
+    if args.cached:
+        dg.alloc_lu_cache()
+        dg.step(use_cache=True, recompute_lu=True)
+        ctx.synchronize()
+        ev0.record(ext)
+        for _ in range(args.steps):
+            dg.step(use_cache=True, recompute_lu=False)
+        ev1.record(ext)
+        ctx.synchronize()
+        torch.cuda.synchronize()
+        cached = 1000.0 * args.steps / ev0.elapsed_time(ev1)
+
+    base = None
+    if not args.no_cpu_baseline:
+        threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+        psd = dg.psd_host()
+        base = cpu_baseline(w, psd, threads, args.cpu_seconds)
+
+    value = 1000.0 / ms_per_step
+    out = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "production grid 29x24x101x91 (R x MLT x E x alpha), 696 band systems of 9191 "
+                               "unknowns (kl=ku=102), Kp=2, dt=300 s, LU caching off",
+                   "grid": [w.nR, w.nP, w.nV, w.nK], "lu_caching": False},
+        "roofline": roofline,
+        "cpu_baseline": base,
+        "phases": phase_out,
+    }
+    if cached is not None:
+        out["lu_cached_steps_per_s"] = round(cached, 4)
+    if base:
+        out["speedup_vs_cpu"] = round(value / base["value"], 2)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,174 @@
+/*
+ * verb4d_hip.h - C-ABI of the MI355X-native VERB-4D operator-split step.
+ *
+ * Plain pointers and sizes only; no C++ or torch types cross this boundary.
+ * Every entry returns an int status:
+ *     0   success
+ *    >0   LAPACK-style info: the 1-based column of the first exactly-zero
+ *         pivot of a band factorisation (dgbtrf's info > 0)
+ *    <0   a verb_status_t error below
+ * The C++ drop-in translation units map status != 0 to the reference's
+ * behaviour (Logger::error + exit(EXIT_FAILURE)); this layer never exits.
+ *
+ * Array conventions follow the reference's Matrix containers (row-major,
+ * last index fastest): 2-D [q1][q2], 4-D [P][R][V][K] (src/Matrix.cpp:161,
+ * 4266). Boundary types use the values of the reference's
+ * BoundaryConditionType (src/BoundaryConditionType.hpp:53-88).
+ *
+ * Threading: a verb_ctx owns one HIP stream and its scratch; calls on one ctx
+ * are serialised by an internal mutex, so the drop-in shims may be called
+ * from OpenMP workers the way the reference time loop does
+ * (src/VERB4D_Solver.cpp:885,1009,1101). Use one ctx per thread for overlap.
+ */
+#ifndef VERB4D_HIP_H
+#define VERB4D_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    VERB_OK = 0,
+    VERB_ERR_ARG = -1,        /* bad argument / unsupported size */
+    VERB_ERR_BC = -2,         /* unknown boundary type (MatrixSolver.cpp:57-58) */
+    VERB_ERR_COURANT = -3,    /* Courant number > 1 (QUICKEST6.cpp:48-52) */
+    VERB_ERR_HIP = -4,        /* HIP runtime failure */
+    VERB_ERR_NOMEM = -5,      /* device allocation failed */
+    VERB_ERR_NODEVICE = -6    /* no HIP device visible */
+} verb_status_t;
+
+/* Values of the reference's BoundaryConditionType enum class. */
+enum { VERB_BC_PERIODIC = 0, VERB_BC_CONSTANT_VALUE = 1, VERB_BC_CONSTANT_DERIVATIVE = 2 };
+
+typedef struct verb_ctx verb_ctx;
+
+/* Context: binds the calling thread's current HIP device, creates a stream. */
+int verb_ctx_create(verb_ctx** out);
+int verb_ctx_destroy(verb_ctx* ctx);
+/* The ctx's stream (a hipStream_t) for callers that enqueue their own work. */
+void* verb_ctx_stream(verb_ctx* ctx);
+int verb_ctx_synchronize(verb_ctx* ctx);
+/* Human-readable text of the last error raised on this ctx. */
+const char* verb_ctx_last_error(verb_ctx* ctx);
+/* Bytes of device scratch the ctx currently holds. */
+size_t verb_ctx_scratch_bytes(verb_ctx* ctx);
+
+/* Kernel-phase timing with HIP events recorded on the ctx stream around each
+ * batched launch (off by default). Phases: VERB_PHASE_*. get_timings
+ * synchronises the stream, writes the summed milliseconds and launch counts
+ * of every phase (arrays of VERB_NUM_PHASES) and clears the totals.
+ * No reference counterpart: the reference only times the whole run
+ * (src/VERB4D_Solver.cpp:429-431, 1311-1312). */
+enum { VERB_PHASE_CONVECTION = 0, VERB_PHASE_RADIAL = 1, VERB_PHASE_LOCAL_ASSEMBLE = 2, VERB_PHASE_LOCAL_FACTOR = 3,
+       VERB_PHASE_LOCAL_SOLVE = 4, VERB_NUM_PHASES = 5 };
+int verb_ctx_set_timing(verb_ctx* ctx, int enable);
+int verb_ctx_get_timings(verb_ctx* ctx, double* ms, long* counts);
+
+/* ------------------------------------------------------------------------ */
+/* Per-call drop-in entries (host pointers, synchronous).                    */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces Diffusion_2D (src/Diffusion_2D.h:86-111, both the plain and the
+ * LU_CACHING variants) including its AddBoundaries_2D /
+ * SecondDerivativeApproximation_2D assembly (src/Diffusion_2D.cpp:43-130)
+ * and the Lapack() banded solve (src/MatrixSolver.cpp:223-340).
+ * 2-D arrays are [nx][ny]; xl/xu have ny entries, yl/yu nx entries.
+ * lu_cache == NULL selects the plain variant. Otherwise lu_cache holds
+ * (2kl+ku+1)*nx*ny doubles and ipiv_cache nx*ny int64 in LAPACK band
+ * layout (kl = ku = nx+1), factored when recompute_lu != 0 and reused
+ * otherwise, as the reference's caller-owned caches are. */
+int verb_diffusion_2d(verb_ctx* ctx, double* psd, const double* x, const double* y, int nx, int ny,
+                      const double* xl, const double* xu, const double* yl, const double* yu,
+                      int xl_type, int xu_type, int yl_type, int yu_type,
+                      const double* Dxx, const double* Dyy, const double* Dxy, const double* Dyx,
+                      const double* G, const double* sources, const double* losses, double dt, double sub_dt,
+                      double* lu_cache, int64_t* ipiv_cache, int recompute_lu);
+
+/* Replaces Diffusion_1D (src/Diffusion_1D.h:69-77): assembly of
+ * getCalculationMatrix1DDiffusion (src/Diffusion_1D.cpp:91-184), RHS
+ * C + B*f and tridag (src/MatrixSolver.cpp:689-720). */
+int verb_diffusion_1d(verb_ctx* ctx, double* psd, const double* x, int n, double lbc, double ubc, int l_type,
+                      int u_type, const double* Dxx, const double* G, const double* sources, const double* losses,
+                      double dt);
+
+/* Replaces Convection_2D (src/Convection_2D.h:98-108, SAVE_PSD_LOST_CONV
+ * off): 2-D arrays [nP][nR]; Pl/Pu have nR entries, Rl/Ru nP entries. */
+int verb_convection_2d(verb_ctx* ctx, double* psd, const double* P, const double* R, int nP, int nR,
+                       const double* Pl, const double* Pu, const double* Rl, const double* Ru,
+                       int Pl_type, int Pu_type, int Rl_type, int Ru_type,
+                       const double* VP, const double* VR, const double* sources, const double* losses,
+                       const double* G_conv, double dt);
+
+/* Replaces Convection_1D_ULTIMATE_QUICKEST6
+ * (src/Convection_1D_ULTIMATE_QUICKEST6.h:71-79). */
+int verb_convection_1d_quickest6(verb_ctx* ctx, double* psd, const double* x, int n, double lbc, double ubc,
+                                 int l_type, int u_type, const double* Ux, double dt);
+
+/* Replace the LAPACK calls the reference binds (src/MatrixSolver.h:243-248):
+ * dgbtrf_ (band LU with partial pivoting, ipiv 1-based, return = info) and
+ * dgbtrs_('N', nrhs = 1). Column-major band storage, ldab >= 2kl+ku+1. */
+int verb_dgbtrf(verb_ctx* ctx, int64_t m, int64_t n, int64_t kl, int64_t ku, double* ab, int64_t ldab,
+                int64_t* ipiv);
+int verb_dgbtrs(verb_ctx* ctx, int64_t n, int64_t kl, int64_t ku, const double* ab, int64_t ldab,
+                const int64_t* ipiv, double* b);
+
+/* Replaces tridag (src/MatrixSolver.cpp:689-720) for `lines` independent
+ * systems stored line-major (line l occupies [l*n, (l+1)*n)). */
+int verb_tridag_batched(verb_ctx* ctx, const double* a, const double* b, const double* c, const double* r,
+                        double* u, int n, int lines);
+
+/* ------------------------------------------------------------------------ */
+/* Batched full-grid entries (device pointers, asynchronous on ctx stream).  */
+/* ------------------------------------------------------------------------ */
+
+/* All fields the reference time loop hands to the three operators, as
+ * device-resident arrays. 4-D fields are [P][R][V][K]. The boundary arrays
+ * are the reference's Matrix3D boundary fields: Pl/Pu [R][V][K],
+ * Rl/Ru [P][V][K] (convection, src/VERB4D_Solver.cpp:909-910),
+ * Ll/Lu [P][V][K] (radial, :1041), Vl/Vu [P][R][K] and Kl/Ku [P][R][V]
+ * (local, :1147-1148). */
+typedef struct verb_grid {
+    int nP, nR, nV, nK;
+    const double *Pg, *Rg, *Vg, *Kg, *Lg;
+    const double *DLL, *DVV, *DKK, *DVK, *VP, *VR;
+    const double *G_local, *G_radial, *G_conv, *sources, *losses_local, *losses_radial, *losses_conv;
+    const double *Pl, *Pu, *Rl, *Ru, *Ll, *Lu, *Vl, *Vu, *Kl, *Ku;
+    int Pl_type, Pu_type, Rl_type, Ru_type, Ll_type, Lu_type, Vl_type, Vu_type, Kl_type, Ku_type;
+    double dt, sub_dt;
+    /* 0/1 multipliers routing the sources to radial or local diffusion
+     * (src/VERB4D_Solver.cpp:420-426, 1037, 1151) */
+    double radial_sources, local_sources;
+    /* include_boundary == 0 skips local diffusion at iR = 0 and nR-1
+     * (src/VERB4D_Solver.cpp:1117-1120) */
+    int include_boundary;
+} verb_grid;
+
+/* Convection_2D over every (V,K) plane (src/VERB4D_Solver.cpp:860-958). */
+int verb_convection_batched(verb_ctx* ctx, const verb_grid* g, double* psd);
+/* Diffusion_1D over every (P,V,K) line (src/VERB4D_Solver.cpp:999-1055). */
+int verb_radial_diffusion_batched(verb_ctx* ctx, const verb_grid* g, double* psd);
+/* Diffusion_2D over every (P,R) node (src/VERB4D_Solver.cpp:1100-1210).
+ * lu_cache/ipiv_cache: NULL for the plain path (factor every call), or
+ * device buffers of verb_lu_cache_doubles / verb_lu_cache_ints elements
+ * (LAPACK band layout per node) factored when recompute_lu != 0. */
+int verb_local_diffusion_batched(verb_ctx* ctx, const verb_grid* g, double* psd, double* lu_cache,
+                                 int64_t* ipiv_cache, int recompute_lu);
+size_t verb_lu_cache_doubles(const verb_grid* g);
+size_t verb_lu_cache_ints(const verb_grid* g);
+
+/* One full operator-split step: convection, radial, local
+ * (src/VERB4D_Solver.cpp:851-1216), each enabled by its run_* flag. */
+int verb_step(verb_ctx* ctx, const verb_grid* g, double* psd, int run_convection, int run_radial, int run_local,
+              double* lu_cache, int64_t* ipiv_cache, int recompute_lu);
+
+/* Pulls the device-side status word of the last batched launches (Courant
+ * violation, zero pivot) and synchronises the stream. */
+int verb_check_status(verb_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VERB4D_HIP_H */

@@ -1,0 +1,539 @@
+// C-ABI of the VERB-4D step library: contexts, per-call drop-in entries
+// (host pointers, synchronous) and batched full-grid entries (device
+// pointers, asynchronous on the context's stream).
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "common.h"
+
+namespace verb {
+int launch_convection_mode(const verb_grid& g, double* psd, int* status, hipStream_t stream, int single_outer);
+}
+
+struct verb_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int* status = nullptr;  // device status words
+    std::mutex mu;
+    std::string last_error;
+    // grow-only device scratch for the per-call entries and the plain
+    // (uncached) local-diffusion path
+    std::vector<std::pair<void*, size_t>> bufs;
+    size_t scratch_bytes = 0;
+    // kernel-phase timing
+    bool timing = false;
+    struct Mark { int phase; hipEvent_t a, b; };
+    std::vector<Mark> marks;
+    double ms[VERB_NUM_PHASES] = {0};
+    long counts[VERB_NUM_PHASES] = {0};
+};
+
+namespace {
+
+int set_err(verb_ctx* c, int code, const char* what) {
+    if (c) c->last_error = what;
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                          \
+    do {                                                                           \
+        hipError_t e_ = (call);                                                    \
+        if (e_ != hipSuccess) return set_err(ctx, VERB_ERR_HIP, hipGetErrorString(e_)); \
+    } while (0)
+
+// Scratch slot `slot` of at least `bytes` bytes (contents not preserved).
+void* scratch(verb_ctx* c, int slot, size_t bytes) {
+    if ((int)c->bufs.size() <= slot) c->bufs.resize(slot + 1, {nullptr, 0});
+    auto& b = c->bufs[slot];
+    if (b.second < bytes) {
+        if (b.first) hipFree(b.first);
+        c->scratch_bytes -= b.second;
+        b.first = nullptr;
+        b.second = 0;
+        if (hipMalloc(&b.first, bytes) != hipSuccess) return nullptr;
+        b.second = bytes;
+        c->scratch_bytes += bytes;
+    }
+    return b.first;
+}
+
+int reset_status(verb_ctx* c) {
+    int init[verb::kStatusWords] = {0, 0x7fffffff, -1, 0};
+    HIPCHK(c, hipMemcpyAsync(c->status, init, sizeof(init), hipMemcpyHostToDevice, c->stream));
+    return VERB_OK;
+}
+
+// Synchronise and translate the device status words into a return code.
+int finish(verb_ctx* c) {
+    int st[verb::kStatusWords];
+    HIPCHK(c, hipMemcpyAsync(st, c->status, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (st[0] == VERB_ERR_COURANT) return set_err(c, VERB_ERR_COURANT, "max(CourNum) > 1, calculation can't be performed");
+    if (st[0] == VERB_ERR_BC) return set_err(c, VERB_ERR_BC, "unknown boundary type");
+    if (st[0] != 0) return set_err(c, st[0], "device error");
+    if (st[1] != 0x7fffffff) return set_err(c, st[1], "Error computing LU decomposition (zero pivot)");
+    return VERB_OK;
+}
+
+bool bc_ok(int t) { return t >= 0 && t <= 2; }
+
+// Upload `count` doubles to scratch slot `slot`.
+const double* up(verb_ctx* c, int slot, const double* h, size_t count, int& rc) {
+    if (rc) return nullptr;
+    if (!h) { rc = set_err(c, VERB_ERR_ARG, "null input array"); return nullptr; }
+    double* d = (double*)scratch(c, slot, count * sizeof(double));
+    if (!d) { rc = set_err(c, VERB_ERR_NOMEM, "device allocation failed"); return nullptr; }
+    if (hipMemcpyAsync(d, h, count * sizeof(double), hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        rc = set_err(c, VERB_ERR_HIP, "host to device copy failed");
+    return d;
+}
+
+struct LocalBuffers {
+    verb::LocalWork w;
+    int rc = 0;
+};
+
+LocalBuffers local_buffers(verb_ctx* c, const verb_grid* g, double* lu_cache, int64_t* ipiv_cache, int base_slot) {
+    LocalBuffers lb;
+    const size_t n = (size_t)g->nV * g->nK;
+    const size_t nodes = (size_t)g->nP * g->nR;
+    const size_t ldab = 3 * (size_t)(g->nV + 1) + 1;
+    if (lu_cache) {
+        lb.w.ab = lu_cache;
+        lb.w.ipiv = ipiv_cache;
+    } else {
+        lb.w.ab = (double*)scratch(c, base_slot + 0, ldab * n * nodes * sizeof(double));
+        lb.w.ipiv = (int64_t*)scratch(c, base_slot + 1, n * nodes * sizeof(int64_t));
+    }
+    lb.w.b0 = (double*)scratch(c, base_slot + 2, n * nodes * sizeof(double));
+    lb.w.c0 = (double*)scratch(c, base_slot + 3, n * nodes * sizeof(double));
+    lb.w.rhs = (double*)scratch(c, base_slot + 4, n * nodes * sizeof(double));
+    if (!lb.w.ab || !lb.w.ipiv || !lb.w.b0 || !lb.w.c0 || !lb.w.rhs)
+        lb.rc = set_err(c, VERB_ERR_NOMEM, "device allocation failed");
+    return lb;
+}
+
+constexpr int kSlotLocal = 40;  // scratch slots 40..44 for the local workspace
+
+hipEvent_t new_event() {
+    hipEvent_t e = nullptr;
+    hipEventCreate(&e);
+    return e;
+}
+
+// Records a start event for `phase` when timing is on; returns its index.
+int mark_begin(verb_ctx* c, int phase) {
+    if (!c->timing) return -1;
+    verb_ctx::Mark m{phase, new_event(), new_event()};
+    hipEventRecord(m.a, c->stream);
+    c->marks.push_back(m);
+    return (int)c->marks.size() - 1;
+}
+
+void mark_end(verb_ctx* c, int idx) {
+    if (idx >= 0) hipEventRecord(c->marks[idx].b, c->stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+int verb_ctx_create(verb_ctx** out) {
+    if (!out) return VERB_ERR_ARG;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count < 1) return VERB_ERR_NODEVICE;
+    verb_ctx* c = new verb_ctx();
+    hipGetDevice(&c->device);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return VERB_ERR_HIP;
+    }
+    if (hipMalloc(&c->status, sizeof(int) * verb::kStatusWords) != hipSuccess) {
+        hipStreamDestroy(c->stream);
+        delete c;
+        return VERB_ERR_NOMEM;
+    }
+    if (reset_status(c) != VERB_OK || hipStreamSynchronize(c->stream) != hipSuccess) {
+        hipFree(c->status);
+        hipStreamDestroy(c->stream);
+        delete c;
+        return VERB_ERR_HIP;
+    }
+    *out = c;
+    return VERB_OK;
+}
+
+int verb_ctx_destroy(verb_ctx* c) {
+    if (!c) return VERB_ERR_ARG;
+    hipStreamSynchronize(c->stream);
+    for (auto& b : c->bufs)
+        if (b.first) hipFree(b.first);
+    hipFree(c->status);
+    hipStreamDestroy(c->stream);
+    delete c;
+    return VERB_OK;
+}
+
+void* verb_ctx_stream(verb_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int verb_ctx_synchronize(verb_ctx* c) {
+    if (!c) return VERB_ERR_ARG;
+    return hipStreamSynchronize(c->stream) == hipSuccess ? VERB_OK : VERB_ERR_HIP;
+}
+
+const char* verb_ctx_last_error(verb_ctx* c) { return c ? c->last_error.c_str() : "null context"; }
+
+size_t verb_ctx_scratch_bytes(verb_ctx* c) { return c ? c->scratch_bytes : 0; }
+
+int verb_ctx_set_timing(verb_ctx* c, int enable) {
+    if (!c) return VERB_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->timing = enable != 0;
+    return VERB_OK;
+}
+
+int verb_ctx_get_timings(verb_ctx* c, double* ms, long* counts) {
+    if (!c) return VERB_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (auto& m : c->marks) {
+        float t = 0.f;
+        if (hipEventElapsedTime(&t, m.a, m.b) == hipSuccess) {
+            c->ms[m.phase] += t;
+            c->counts[m.phase] += 1;
+        }
+        hipEventDestroy(m.a);
+        hipEventDestroy(m.b);
+    }
+    c->marks.clear();
+    for (int i = 0; i < VERB_NUM_PHASES; i++) {
+        if (ms) ms[i] = c->ms[i];
+        if (counts) counts[i] = c->counts[i];
+        c->ms[i] = 0;
+        c->counts[i] = 0;
+    }
+    return VERB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batched entries
+// ---------------------------------------------------------------------------
+int verb_check_status(verb_ctx* c) {
+    if (!c) return VERB_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = finish(c);
+    int r2 = reset_status(c);
+    hipStreamSynchronize(c->stream);
+    return rc ? rc : r2;
+}
+
+static int grid_ok(const verb_grid* g) {
+    if (!g || g->nP < 1 || g->nR < 1 || g->nV < 1 || g->nK < 1) return 0;
+    if (!(g->dt > 0)) return 0;
+    return 1;
+}
+
+int verb_convection_batched(verb_ctx* c, const verb_grid* g, double* psd) {
+    if (!c || !grid_ok(g) || !psd) return VERB_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    const int m = mark_begin(c, VERB_PHASE_CONVECTION);
+    int rc = verb::launch_convection(*g, psd, c->status, c->stream);
+    mark_end(c, m);
+    return rc ? set_err(c, rc, "convection launch rejected") : VERB_OK;
+}
+
+int verb_radial_diffusion_batched(verb_ctx* c, const verb_grid* g, double* psd) {
+    if (!c || !grid_ok(g) || !psd) return VERB_ERR_ARG;
+    if (!bc_ok(g->Ll_type) || !bc_ok(g->Lu_type)) return VERB_ERR_BC;
+    std::lock_guard<std::mutex> lk(c->mu);
+    const int m = mark_begin(c, VERB_PHASE_RADIAL);
+    int rc = verb::launch_radial(*g, psd, c->status, c->stream);
+    mark_end(c, m);
+    return rc ? set_err(c, rc, "radial launch rejected") : VERB_OK;
+}
+
+size_t verb_lu_cache_doubles(const verb_grid* g) {
+    if (!g) return 0;
+    const size_t n = (size_t)g->nV * g->nK;
+    return (3 * (size_t)(g->nV + 1) + 1) * n * (size_t)g->nP * g->nR;
+}
+
+size_t verb_lu_cache_ints(const verb_grid* g) {
+    if (!g) return 0;
+    return (size_t)g->nV * g->nK * (size_t)g->nP * g->nR;
+}
+
+int verb_local_diffusion_batched(verb_ctx* c, const verb_grid* g, double* psd, double* lu_cache,
+                                 int64_t* ipiv_cache, int recompute_lu) {
+    if (!c || !grid_ok(g) || !psd) return VERB_ERR_ARG;
+    if ((lu_cache == nullptr) != (ipiv_cache == nullptr)) return VERB_ERR_ARG;
+    if (!(g->sub_dt > 0)) return VERB_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    LocalBuffers lb = local_buffers(c, g, lu_cache, ipiv_cache, kSlotLocal);
+    if (lb.rc) return lb.rc;
+    const bool factor = (lu_cache == nullptr) || recompute_lu;
+    hipEvent_t ev[6];
+    hipEvent_t* marks = nullptr;
+    if (c->timing) {
+        for (auto& e : ev) e = new_event();
+        marks = ev;
+    }
+    int rc = verb::launch_local(*g, psd, lb.w, factor, c->status, c->stream, marks);
+    if (marks) {
+        c->marks.push_back({VERB_PHASE_LOCAL_ASSEMBLE, ev[0], ev[1]});
+        c->marks.push_back({VERB_PHASE_LOCAL_FACTOR, ev[2], ev[3]});
+        c->marks.push_back({VERB_PHASE_LOCAL_SOLVE, ev[4], ev[5]});
+    }
+    return rc ? set_err(c, rc, "local diffusion launch rejected") : VERB_OK;
+}
+
+int verb_step(verb_ctx* c, const verb_grid* g, double* psd, int run_convection, int run_radial, int run_local,
+              double* lu_cache, int64_t* ipiv_cache, int recompute_lu) {
+    int rc = VERB_OK;
+    if (run_convection && (3 < g->nP || 3 < g->nR)) rc = verb_convection_batched(c, g, psd);
+    if (!rc && run_radial && g->nR >= 3) rc = verb_radial_diffusion_batched(c, g, psd);
+    if (!rc && run_local && g->nV >= 3 && g->nK >= 3)
+        rc = verb_local_diffusion_batched(c, g, psd, lu_cache, ipiv_cache, recompute_lu);
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Per-call drop-in entries
+// ---------------------------------------------------------------------------
+int verb_diffusion_2d(verb_ctx* c, double* psd, const double* x, const double* y, int nx, int ny,
+                      const double* xl, const double* xu, const double* yl, const double* yu, int xl_type,
+                      int xu_type, int yl_type, int yu_type, const double* Dxx, const double* Dyy,
+                      const double* Dxy, const double* Dyx, const double* G, const double* sources,
+                      const double* losses, double dt, double sub_dt, double* lu_cache, int64_t* ipiv_cache,
+                      int recompute_lu) {
+    if (!c || !psd || nx < 3 || ny < 3 || !(sub_dt > 0)) return VERB_ERR_ARG;
+    if (!bc_ok(xl_type) || !bc_ok(xu_type) || !bc_ok(yl_type) || !bc_ok(yu_type)) return VERB_ERR_BC;
+    if (Dxy != Dyx) {
+        // The device assembly reads one mixed coefficient for both mixed
+        // terms, as the reference time loop passes DVK twice.
+        const size_t n = (size_t)nx * ny;
+        for (size_t i = 0; i < n; i++)
+            if (Dxy[i] != Dyx[i]) return set_err(c, VERB_ERR_ARG, "Dxy != Dyx is not supported");
+    }
+    std::lock_guard<std::mutex> lk(c->mu);
+    const size_t n = (size_t)nx * ny;
+    int rc = 0;
+    verb_grid g;
+    std::memset(&g, 0, sizeof(g));
+    g.nP = 1; g.nR = 1; g.nV = nx; g.nK = ny;
+    double* dpsd = (double*)scratch(c, 0, n * sizeof(double));
+    if (!dpsd) return set_err(c, VERB_ERR_NOMEM, "device allocation failed");
+    HIPCHK(c, hipMemcpyAsync(dpsd, psd, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    g.Vg = up(c, 1, x, n, rc);
+    g.Kg = up(c, 2, y, n, rc);
+    g.Vl = up(c, 3, xl, ny, rc);
+    g.Vu = up(c, 4, xu, ny, rc);
+    g.Kl = up(c, 5, yl, nx, rc);
+    g.Ku = up(c, 6, yu, nx, rc);
+    g.DVV = up(c, 7, Dxx, n, rc);
+    g.DKK = up(c, 8, Dyy, n, rc);
+    g.DVK = up(c, 9, Dxy, n, rc);
+    g.G_local = up(c, 10, G, n, rc);
+    g.sources = up(c, 11, sources, n, rc);
+    g.losses_local = up(c, 12, losses, n, rc);
+    if (rc) return rc;
+    g.Vl_type = xl_type; g.Vu_type = xu_type; g.Kl_type = yl_type; g.Ku_type = yu_type;
+    g.dt = dt; g.sub_dt = sub_dt; g.local_sources = 1.0; g.include_boundary = 1;
+    const int kl = nx + 1, ldab = 3 * kl + 1;
+    double* dab = nullptr;
+    int64_t* dpiv = nullptr;
+    if (lu_cache) {
+        dab = (double*)scratch(c, 13, (size_t)ldab * n * sizeof(double));
+        dpiv = (int64_t*)scratch(c, 14, n * sizeof(int64_t));
+        if (!dab || !dpiv) return set_err(c, VERB_ERR_NOMEM, "device allocation failed");
+        if (!recompute_lu) {
+            HIPCHK(c, hipMemcpyAsync(dab, lu_cache, (size_t)ldab * n * sizeof(double), hipMemcpyHostToDevice,
+                                     c->stream));
+            HIPCHK(c, hipMemcpyAsync(dpiv, ipiv_cache, n * sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+        }
+    }
+    LocalBuffers lb = local_buffers(c, &g, dab, dpiv, 20);
+    if (lb.rc) return lb.rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if ((rc = reset_status(c))) return rc;
+    const bool factor = (lu_cache == nullptr) || recompute_lu;
+    rc = verb::launch_local(g, dpsd, lb.w, factor, c->status, c->stream);
+    if (rc) return set_err(c, rc, "local diffusion launch rejected");
+    rc = finish(c);
+    reset_status(c);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(psd, dpsd, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (lu_cache && recompute_lu) {
+        HIPCHK(c, hipMemcpyAsync(lu_cache, lb.w.ab, (size_t)ldab * n * sizeof(double), hipMemcpyDeviceToHost,
+                                 c->stream));
+        HIPCHK(c, hipMemcpyAsync(ipiv_cache, lb.w.ipiv, n * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return VERB_OK;
+}
+
+int verb_diffusion_1d(verb_ctx* c, double* psd, const double* x, int n, double lbc, double ubc, int l_type,
+                      int u_type, const double* Dxx, const double* G, const double* sources, const double* losses,
+                      double dt) {
+    if (!c || !psd || n < 3 || n > 64) return VERB_ERR_ARG;
+    if (!bc_ok(l_type) || !bc_ok(u_type)) return VERB_ERR_BC;
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = 0;
+    verb_grid g;
+    std::memset(&g, 0, sizeof(g));
+    g.nP = 1; g.nR = n; g.nV = 1; g.nK = 1;
+    double* dpsd = (double*)scratch(c, 0, n * sizeof(double));
+    if (!dpsd) return set_err(c, VERB_ERR_NOMEM, "device allocation failed");
+    HIPCHK(c, hipMemcpyAsync(dpsd, psd, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    g.Lg = up(c, 1, x, n, rc);
+    g.DLL = up(c, 2, Dxx, n, rc);
+    g.G_radial = up(c, 3, G, n, rc);
+    g.sources = up(c, 4, sources, n, rc);
+    g.losses_radial = up(c, 5, losses, n, rc);
+    g.Ll = up(c, 6, &lbc, 1, rc);
+    g.Lu = up(c, 7, &ubc, 1, rc);
+    if (rc) return rc;
+    g.Ll_type = l_type; g.Lu_type = u_type; g.dt = dt; g.radial_sources = 1.0;
+    if ((rc = reset_status(c))) return rc;
+    rc = verb::launch_radial(g, dpsd, c->status, c->stream);
+    if (rc) return set_err(c, rc, "radial launch rejected");
+    rc = finish(c);
+    reset_status(c);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(psd, dpsd, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return VERB_OK;
+}
+
+static int convection_call(verb_ctx* c, double* psd, const double* P, const double* R, int nP, int nR,
+                           const double* Pl, const double* Pu, const double* Rl, const double* Ru, int Pl_type,
+                           int Pu_type, int Rl_type, int Ru_type, const double* VP, const double* VR,
+                           const double* sources, const double* losses, const double* G, double dt,
+                           int single_outer) {
+    std::lock_guard<std::mutex> lk(c->mu);
+    const size_t np = (size_t)nP * nR;
+    int rc = 0;
+    verb_grid g;
+    std::memset(&g, 0, sizeof(g));
+    g.nP = nP; g.nR = nR; g.nV = 1; g.nK = 1;
+    double* dpsd = (double*)scratch(c, 0, np * sizeof(double));
+    if (!dpsd) return set_err(c, VERB_ERR_NOMEM, "device allocation failed");
+    HIPCHK(c, hipMemcpyAsync(dpsd, psd, np * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    g.Pg = up(c, 1, P, np, rc);
+    g.Rg = up(c, 2, R, np, rc);
+    g.VP = up(c, 3, VP, np, rc);
+    g.VR = up(c, 4, VR, np, rc);
+    g.sources = up(c, 5, sources, np, rc);
+    g.losses_conv = up(c, 6, losses, np, rc);
+    g.G_conv = up(c, 7, G, np, rc);
+    if (Pl_type != VERB_BC_PERIODIC) g.Pl = up(c, 8, Pl, nR, rc);
+    if (Pu_type != VERB_BC_PERIODIC) g.Pu = up(c, 9, Pu, nR, rc);
+    if (Rl_type != VERB_BC_PERIODIC) g.Rl = up(c, 10, Rl, nP, rc);
+    if (Ru_type != VERB_BC_PERIODIC) g.Ru = up(c, 11, Ru, nP, rc);
+    if (rc) return rc;
+    g.Pl_type = Pl_type; g.Pu_type = Pu_type; g.Rl_type = Rl_type; g.Ru_type = Ru_type;
+    g.dt = dt;
+    if ((rc = reset_status(c))) return rc;
+    rc = verb::launch_convection_mode(g, dpsd, c->status, c->stream, single_outer);
+    if (rc) return set_err(c, rc, "convection launch rejected");
+    rc = finish(c);
+    reset_status(c);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(psd, dpsd, np * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return VERB_OK;
+}
+
+int verb_convection_2d(verb_ctx* c, double* psd, const double* P, const double* R, int nP, int nR,
+                       const double* Pl, const double* Pu, const double* Rl, const double* Ru, int Pl_type,
+                       int Pu_type, int Rl_type, int Ru_type, const double* VP, const double* VR,
+                       const double* sources, const double* losses, const double* G_conv, double dt) {
+    if (!c || !psd || nP < 1 || nR < 1) return VERB_ERR_ARG;
+    if (!bc_ok(Pl_type) || !bc_ok(Pu_type) || !bc_ok(Rl_type) || !bc_ok(Ru_type)) return VERB_ERR_BC;
+    return convection_call(c, psd, P, R, nP, nR, Pl, Pu, Rl, Ru, Pl_type, Pu_type, Rl_type, Ru_type, VP, VR, sources,
+                           losses, G_conv, dt, 0);
+}
+
+int verb_convection_1d_quickest6(verb_ctx* c, double* psd, const double* x, int n, double lbc, double ubc,
+                                 int l_type, int u_type, const double* Ux, double dt) {
+    if (!c || !psd || n < 6) return VERB_ERR_ARG;
+    if (!bc_ok(l_type) || !bc_ok(u_type)) return VERB_ERR_BC;
+    // A single line is the R sweep of a one-row plane with G = 1, no losses,
+    // no sources and one outer step: x*1, x/1, x*exp(0) and x + 0*dt leave
+    // every value bit-identical, so only the QUICKEST6 update remains.
+    std::vector<double> ones(n, 1.0), zeros(n, 0.0);
+    return convection_call(c, psd, x, x, 1, n, nullptr, nullptr, &lbc, &ubc, VERB_BC_PERIODIC, VERB_BC_PERIODIC,
+                           l_type, u_type, zeros.data(), Ux, zeros.data(), zeros.data(), ones.data(), dt, 1);
+}
+
+int verb_dgbtrf(verb_ctx* c, int64_t m, int64_t n, int64_t kl, int64_t ku, double* ab, int64_t ldab,
+                int64_t* ipiv) {
+    if (!c || !ab || !ipiv || m != n || n < 1 || kl < 0 || ku < 0 || ldab < 2 * kl + ku + 1 || n > (1 << 24))
+        return VERB_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    const size_t bytes = (size_t)ldab * n * sizeof(double);
+    double* dab = (double*)scratch(c, 30, bytes);
+    int64_t* dpiv = (int64_t*)scratch(c, 31, (size_t)n * sizeof(int64_t));
+    if (!dab || !dpiv) return set_err(c, VERB_ERR_NOMEM, "device allocation failed");
+    HIPCHK(c, hipMemcpyAsync(dab, ab, bytes, hipMemcpyHostToDevice, c->stream));
+    int rc = reset_status(c);
+    if (rc) return rc;
+    rc = verb::launch_dgbtrf((int)n, (int)kl, (int)ku, dab, (int)ldab, dpiv, c->status, c->stream);
+    if (rc) return set_err(c, rc, "dgbtrf launch rejected");
+    int st[verb::kStatusWords];
+    HIPCHK(c, hipMemcpyAsync(st, c->status, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(ab, dab, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(ipiv, dpiv, (size_t)n * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    reset_status(c);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return st[1] == 0x7fffffff ? VERB_OK : st[1];
+}
+
+int verb_dgbtrs(verb_ctx* c, int64_t n, int64_t kl, int64_t ku, const double* ab, int64_t ldab,
+                const int64_t* ipiv, double* b) {
+    if (!c || !ab || !ipiv || !b || n < 1 || kl < 0 || ku < 0 || ldab < 2 * kl + ku + 1) return VERB_ERR_ARG;
+    for (int64_t j = 0; j < n; j++)  // interchanges stay inside the band, as dgbtrf produces them
+        if (ipiv[j] < j + 1 || ipiv[j] > std::min<int64_t>(n, j + 1 + kl)) return VERB_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    const size_t bytes = (size_t)ldab * n * sizeof(double);
+    double* dab = (double*)scratch(c, 30, bytes);
+    int64_t* dpiv = (int64_t*)scratch(c, 31, (size_t)n * sizeof(int64_t));
+    double* db = (double*)scratch(c, 32, (size_t)n * sizeof(double));
+    if (!dab || !dpiv || !db) return set_err(c, VERB_ERR_NOMEM, "device allocation failed");
+    HIPCHK(c, hipMemcpyAsync(dab, ab, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(dpiv, ipiv, (size_t)n * sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(db, b, (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    int rc = verb::launch_dgbtrs((int)n, (int)kl, (int)ku, dab, (int)ldab, dpiv, db, c->stream);
+    if (rc) return set_err(c, rc, "dgbtrs launch rejected");
+    HIPCHK(c, hipMemcpyAsync(b, db, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return VERB_OK;
+}
+
+int verb_tridag_batched(verb_ctx* c, const double* a, const double* b, const double* cc, const double* r, double* u,
+                        int n, int lines) {
+    if (!c || !a || !b || !cc || !r || !u || n < 1 || n > 64 || lines < 0) return VERB_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    const size_t cnt = (size_t)n * lines;
+    int rc = 0;
+    const double* da = up(c, 1, a, cnt, rc);
+    const double* db = up(c, 2, b, cnt, rc);
+    const double* dc = up(c, 3, cc, cnt, rc);
+    const double* dr = up(c, 4, r, cnt, rc);
+    if (rc) return rc;
+    double* du = (double*)scratch(c, 5, cnt * sizeof(double));
+    if (!du) return set_err(c, VERB_ERR_NOMEM, "device allocation failed");
+    rc = verb::launch_tridag(da, db, dc, dr, du, n, lines, c->stream);
+    if (rc) return set_err(c, rc, "tridag launch rejected");
+    HIPCHK(c, hipMemcpyAsync(u, du, cnt * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return VERB_OK;
+}
+
+}  // extern "C"

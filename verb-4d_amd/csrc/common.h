@@ -1,0 +1,48 @@
+// Shared device/host declarations of the MI355X VERB-4D step library.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "verb4d_hip.h"
+
+namespace verb {
+
+// Device status word layout (int32 array owned by each ctx):
+//   [0] first error code raised by a kernel (0 = none), e.g. VERB_ERR_COURANT
+//   [1] smallest 1-based zero-pivot column seen by a band factorisation
+//       (INT32_MAX = none)
+//   [2] node index of that pivot
+constexpr int kStatusWords = 4;
+
+__device__ inline void raise_error(int* status, int code) { atomicCAS(&status[0], 0, code); }
+
+__device__ inline void raise_pivot(int* status, int col1, int node) {
+    int old = atomicMin(&status[1], col1);
+    if (col1 < old) atomicExch(&status[2], node);
+}
+
+// Launchers (defined in the .hip files). All asynchronous on `stream`.
+int launch_radial(const verb_grid& g, double* psd, int* status, hipStream_t stream);
+int launch_convection(const verb_grid& g, double* psd, int* status, hipStream_t stream);
+
+struct LocalWork {
+    double* ab;       // band factors, per node (2kl+ku+1)*n doubles (LAPACK layout)
+    int64_t* ipiv;    // per node n pivots, 1-based
+    double* b0;       // per node n: diagonal of B (energy-fast order)
+    double* c0;       // per node n: diagonal of C
+    double* rhs;      // per node n: right-hand side / solution (energy-fast order)
+};
+// marks (nullable, 6 events): begin/end of the assembly, of the band factor
+// and of the solve (+ scatter), recorded on `stream`.
+int launch_local(const verb_grid& g, double* psd, const LocalWork& w, bool factor, int* status,
+                 hipStream_t stream, hipEvent_t* marks = nullptr);
+
+// Single-system LAPACK band routines on device buffers (one system).
+int launch_dgbtrf(int n, int kl, int ku, double* ab, int ldab, int64_t* ipiv, int* status, hipStream_t stream);
+int launch_dgbtrs(int n, int kl, int ku, const double* ab, int ldab, const int64_t* ipiv, double* b,
+                  hipStream_t stream);
+int launch_tridag(const double* a, const double* b, const double* c, const double* r, double* u, int n,
+                  int lines, hipStream_t stream);
+
+}  // namespace verb

@@ -1,0 +1,435 @@
+// Local (E, alpha) diffusion: Diffusion_2D over every (P,R) node.
+//
+// Three kernels per call:
+//   local_assemble  - the 9-point operator of src/Diffusion_2D.cpp:49-130
+//                     (AddBoundaries_2D, src/MatrixSolver.cpp:118-213, and
+//                     twelve SecondDerivativeApproximation_2D calls,
+//                     src/MatrixSolver.cpp:439-494), written straight into
+//                     LAPACK band storage (src/MatrixSolver.cpp:283-299).
+//                     Compiled without FMA contraction: bit-identical to the
+//                     reference assembly.
+//   band_factor     - banded LU with LAPACK's partial pivoting (dgbtf2:
+//                     first-largest |a| among rows j..j+kl of column j,
+//                     row swap over columns j..ju, reciprocal scaling,
+//                     rank-1 update), one workgroup per node.
+//   band_solve      - num_substeps x { rhs = B0*rhs + C0; dgbtrs('N') }
+//                     (src/MatrixSolver.cpp:309-326), one workgroup per node,
+//                     the right-hand side resident in LDS.
+// The unknown vector is energy-fast (in = iK*nV + iV, src/Matrix.cpp:6479),
+// so kl = ku = nV+1 as in the reference.
+#include "common.h"
+
+namespace verb {
+namespace {
+
+constexpr int kAsmThreads = 256;
+
+// Node-level view of the grid slices that Diffusion_2D receives.
+struct NodeArgs {
+    int nx, ny;          // V, K
+    double dt;           // sub-step dt
+    const double *x, *y, *Dxx, *Dyy, *Dxy, *G, *S, *L;  // [nx][ny], base of this node
+    const double *xl, *xu, *yl, *yu;                     // ny, ny, nx, nx
+    int xlt, xut, ylt, yut;
+    double src_mult;
+};
+
+__device__ inline void sda2(double A[9], const NodeArgs& a, int ix, int iy, int dx1, int dy1, int dx2, int dy2,
+                            const double* D, double mult) {
+    const int ny = a.ny;
+#define AT(arr, i, j) (arr)[(size_t)(i) * ny + (j)]
+    const double dh1 = (AT(a.x, ix, iy) + AT(a.y, ix, iy)) - (AT(a.x, ix + dx1, iy + dy1) + AT(a.y, ix + dx1, iy + dy1));
+    const double dh21 = (AT(a.x, ix, iy) + AT(a.y, ix, iy)) - (AT(a.x, ix + dx2, iy + dy2) + AT(a.y, ix + dx2, iy + dy2));
+    const double dh22 = (AT(a.x, ix + dx1, iy + dy1) + AT(a.y, ix + dx1, iy + dy1)) -
+                        (AT(a.x, ix + dx1 + dx2, iy + dy1 + dy2) + AT(a.y, ix + dx1 + dx2, iy + dy1 + dy2));
+    const double common_part = mult / AT(a.G, ix, iy) / dh1;
+    A[4] += +common_part / dh21 * AT(D, ix, iy) * AT(a.G, ix, iy);
+    A[(dy2 + 1) * 3 + (dx2 + 1)] += -common_part / dh21 * AT(D, ix, iy) * AT(a.G, ix, iy);
+    A[(dy1 + 1) * 3 + (dx1 + 1)] += -common_part / dh22 * AT(D, ix + dx1, iy + dy1) * AT(a.G, ix + dx1, iy + dy1);
+    A[(dy1 + dy2 + 1) * 3 + (dx1 + dx2 + 1)] +=
+        +common_part / dh22 * AT(D, ix + dx1, iy + dy1) * AT(a.G, ix + dx1, iy + dy1);
+#undef AT
+}
+
+// Row `in` of A (9 slots, slot (dy+1)*3+(dx+1) for offset dy*nx+dx), B0, C0.
+// Returns false on an unknown boundary type.
+__device__ inline bool assemble_row(const NodeArgs& a, int ix, int iy, double A[9], double& B0, double& C0) {
+#pragma unroll
+    for (int s = 0; s < 9; s++) A[s] = 0.0;
+    B0 = 0.0;
+    C0 = 0.0;
+    const int nx = a.nx, ny = a.ny;
+    if (ix == 0 || ix == nx - 1 || iy == 0 || iy == ny - 1) {
+        int type, s1;
+        double dh;
+        if (ix == 0) {
+            C0 = a.xl[iy]; s1 = 5; dh = a.x[(size_t)(ix + 1) * ny + iy] - a.x[(size_t)ix * ny + iy]; type = a.xlt;
+        } else if (ix == nx - 1) {
+            C0 = a.xu[iy]; s1 = 3; dh = a.x[(size_t)ix * ny + iy] - a.x[(size_t)(ix - 1) * ny + iy]; type = a.xut;
+        } else if (iy == 0) {
+            C0 = a.yl[ix]; s1 = 7; dh = a.y[(size_t)ix * ny + iy + 1] - a.y[(size_t)ix * ny + iy]; type = a.ylt;
+        } else {
+            C0 = a.yu[ix]; s1 = 1; dh = a.y[(size_t)ix * ny + iy] - a.y[(size_t)ix * ny + iy - 1]; type = a.yut;
+        }
+        if (type == VERB_BC_CONSTANT_VALUE) { A[4] = 1; A[s1] = 0; }
+        else if (type == VERB_BC_CONSTANT_DERIVATIVE) { A[4] = 1 / dh; A[s1] = -1 / dh; }
+        else return false;
+        return true;
+    }
+    const size_t p = (size_t)ix * ny + iy;
+    A[4] += 1.0 / a.dt;
+    B0 += 1.0 / a.dt;
+    C0 += a.S[p] * a.src_mult;
+    A[4] -= a.L[p];
+    sda2(A, a, ix, iy, -1, 0, +1, 0, a.Dxx, -0.5);
+    sda2(A, a, ix, iy, +1, 0, -1, 0, a.Dxx, -0.5);
+    sda2(A, a, ix, iy, 0, -1, 0, +1, a.Dyy, -0.5);
+    sda2(A, a, ix, iy, 0, +1, 0, -1, a.Dyy, -0.5);
+    sda2(A, a, ix, iy, 0, -1, +1, 0, a.Dxy, -0.25);
+    sda2(A, a, ix, iy, 0, +1, -1, 0, a.Dxy, -0.25);
+    sda2(A, a, ix, iy, 0, -1, -1, 0, a.Dxy, -0.25);
+    sda2(A, a, ix, iy, 0, +1, +1, 0, a.Dxy, -0.25);
+    sda2(A, a, ix, iy, -1, 0, 0, +1, a.Dxy, -0.25);  // Dyx is DVK too (VERB4D_Solver.cpp:1150)
+    sda2(A, a, ix, iy, +1, 0, 0, -1, a.Dxy, -0.25);
+    sda2(A, a, ix, iy, -1, 0, 0, -1, a.Dxy, -0.25);
+    sda2(A, a, ix, iy, +1, 0, 0, +1, a.Dxy, -0.25);
+    return true;
+}
+
+__device__ inline NodeArgs node_args(const verb_grid& g, int iP, int iR, double dt_sub) {
+    NodeArgs a;
+    a.nx = g.nV; a.ny = g.nK; a.dt = dt_sub;
+    const size_t o = ((size_t)iP * g.nR + iR) * g.nV * g.nK;
+    a.x = g.Vg + o; a.y = g.Kg + o; a.Dxx = g.DVV + o; a.Dyy = g.DKK + o; a.Dxy = g.DVK + o;
+    a.G = g.G_local + o; a.S = g.sources + o; a.L = g.losses_local + o;
+    a.xl = g.Vl + ((size_t)iP * g.nR + iR) * g.nK;
+    a.xu = g.Vu + ((size_t)iP * g.nR + iR) * g.nK;
+    a.yl = g.Kl + ((size_t)iP * g.nR + iR) * g.nV;
+    a.yu = g.Ku + ((size_t)iP * g.nR + iR) * g.nV;
+    a.xlt = g.Vl_type; a.xut = g.Vu_type; a.ylt = g.Kl_type; a.yut = g.Ku_type;
+    a.src_mult = g.local_sources;
+    return a;
+}
+
+__device__ inline bool node_skipped(const verb_grid& g, int iR) {
+    return !g.include_boundary && (iR == 0 || iR == g.nR - 1);
+}
+
+// One thread per grid point of one node (blockIdx.y = node). Writes the
+// nonzero band entries of row `in` (the caller zeroed the band), B0, C0
+// and the right-hand-side start vector psd in energy-fast order.
+__global__ __launch_bounds__(kAsmThreads) void local_assemble(const verb_grid g, const double* __restrict__ psd,
+                                                             double* __restrict__ ab, double* __restrict__ b0,
+                                                             double* __restrict__ c0, double* __restrict__ rhs,
+                                                             double dt_sub, int write_band, int* status) {
+    const int node = blockIdx.y;
+    const int iP = node / g.nR, iR = node % g.nR;
+    if (node_skipped(g, iR)) return;
+    const int nx = g.nV, ny = g.nK, n = nx * ny;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int ix = p / ny, iy = p % ny;
+    const int in = iy * nx + ix;
+    const NodeArgs a = node_args(g, iP, iR, dt_sub);
+    double A[9], B0, C0;
+    if (!assemble_row(a, ix, iy, A, B0, C0)) raise_error(status, VERB_ERR_BC);
+    const size_t nn = (size_t)node * n;
+    b0[nn + in] = B0;
+    c0[nn + in] = C0;
+    rhs[nn + in] = psd[nn + p];
+    if (write_band) {
+        const int kl = nx + 1, ku = nx + 1, ldab = 2 * kl + ku + 1;
+        double* abn = ab + (size_t)node * ldab * n;
+#pragma unroll
+        for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+            for (int dx = -1; dx <= 1; dx++) {
+                const int off = dy * nx + dx;
+                const int col = in + off;
+                if (col >= 0 && col < n) abn[(size_t)col * ldab + (kl + ku - off)] = A[(dy + 1) * 3 + (dx + 1)];
+            }
+    }
+}
+
+__global__ void fill_zero(double* __restrict__ p, size_t count) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < count; i += stride) p[i] = 0.0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Banded LU (dgbtf2 semantics), one workgroup per system.
+// ---------------------------------------------------------------------------
+constexpr int kLuThreads = 256;
+
+__global__ __launch_bounds__(kLuThreads) void band_factor_kernel(double* __restrict__ ab_all,
+                                                                 int64_t* __restrict__ ipiv_all, size_t ab_stride,
+                                                                 int n, int kl, int ku, int ldab, int* status,
+                                                                 int nR, int include_boundary) {
+    const int node = blockIdx.x;
+    if (nR > 0 && !include_boundary && (node % nR == 0 || node % nR == nR - 1)) return;
+    double* ab = ab_all + (size_t)node * ab_stride;
+    int64_t* ipiv = ipiv_all + (size_t)node * n;
+    const int kv = ku + kl;
+    const int tid = threadIdx.x;
+    extern __shared__ double sh[];
+    double* sL = sh;                  // kl + 1
+    double* sU = sh + (kl + 1);       // kl + ku + 1
+    __shared__ int s_jp;
+    __shared__ double s_val[kLuThreads / 64];
+    __shared__ int s_idx[kLuThreads / 64];
+#define AB(i, j) ab[(size_t)(j) * ldab + (i)]
+    // zero the fill-in rows of columns ku+1 .. min(kv,n)-1 (dgbtf2 prologue)
+    for (int j = ku + 1; j <= min(kv, n) - 1; j++)
+        for (int i = kv - j + tid; i < kl; i += kLuThreads) AB(i, j) = 0.0;
+    int ju = 0;
+    int info = 0;
+    __syncthreads();
+    for (int j = 0; j < n; j++) {
+        if (j + kv < n)
+            for (int i = tid; i < kl; i += kLuThreads) AB(i, j + kv) = 0.0;
+        const int km = min(kl, n - 1 - j);
+        // first index of max |AB(kv+i, j)|, i in [0, km]
+        double best = -1.0;
+        int bidx = 0x7fffffff;
+        for (int i = tid; i <= km; i += kLuThreads) {
+            const double v = fabs(AB(kv + i, j));
+            if (v > best) { best = v; bidx = i; }
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            const double ov = __shfl_xor(best, off);
+            const int oi = __shfl_xor(bidx, off);
+            if (ov > best || (ov == best && oi < bidx)) { best = ov; bidx = oi; }
+        }
+        if ((tid & 63) == 0) { s_val[tid >> 6] = best; s_idx[tid >> 6] = bidx; }
+        __syncthreads();
+        if (tid == 0) {
+            double b = s_val[0];
+            int bi = s_idx[0];
+            for (int w = 1; w < kLuThreads / 64; w++)
+                if (s_val[w] > b || (s_val[w] == b && s_idx[w] < bi)) { b = s_val[w]; bi = s_idx[w]; }
+            // no comparable entry (all NaN): idamax returns the first index
+            s_jp = (bi > km) ? 0 : bi;
+        }
+        __syncthreads();
+        const int jp = s_jp;
+        if (tid == 0) ipiv[j] = (int64_t)(jp + j + 1);
+        const double piv = AB(kv + jp, j);
+        if (piv != 0.0) {
+            ju = max(ju, min(j + ku + jp, n - 1));
+            if (jp != 0) {
+                for (int c = j + tid; c <= ju; c += kLuThreads) {
+                    const double t = AB(kv + jp + j - c, c);
+                    AB(kv + jp + j - c, c) = AB(kv + j - c, c);
+                    AB(kv + j - c, c) = t;
+                }
+            }
+            __syncthreads();
+            if (km > 0) {
+                const double r = 1.0 / AB(kv, j);
+                for (int i = 1 + tid; i <= km; i += kLuThreads) {
+                    const double v = AB(kv + i, j) * r;
+                    AB(kv + i, j) = v;
+                    sL[i] = v;
+                }
+                for (int c = j + 1 + tid; c <= ju; c += kLuThreads) sU[c - j] = AB(kv + j - c, c);
+                __syncthreads();
+                const int w = ju - j;
+                const int total = w * km;
+                for (int e = tid; e < total; e += kLuThreads) {
+                    const int cc = e / km;
+                    const int i = 1 + (e - cc * km);
+                    const int c = j + 1 + cc;
+                    const double temp = -sU[cc + 1];
+                    if (temp != 0.0) AB(kv + j + i - c, c) += sL[i] * temp;
+                }
+            }
+        } else if (info == 0) {
+            info = j + 1;
+        }
+        __syncthreads();
+    }
+#undef AB
+    if (tid == 0 && info != 0) raise_pivot(status, info, node);
+}
+
+// ---------------------------------------------------------------------------
+// Band solve: per node num_substeps x { x = B0*x + C0; dgbtrs('N') }.
+// One wavefront per system; the solution vector lives in LDS, band
+// columns are staged through LDS in chunks of kChunk columns.
+// ---------------------------------------------------------------------------
+constexpr int kSolveThreads = 64;
+
+__global__ __launch_bounds__(kSolveThreads) void band_solve_kernel(
+    const double* __restrict__ ab_all, const int64_t* __restrict__ ipiv_all, size_t ab_stride, int n, int kl,
+    int ku, int ldab, const double* __restrict__ b0_all, const double* __restrict__ c0_all,
+    double* __restrict__ rhs_all, int nsub, int chunk, int nR, int include_boundary) {
+    const int node = blockIdx.x;
+    if (nR > 0 && !include_boundary && (node % nR == 0 || node % nR == nR - 1)) return;
+    const double* ab = ab_all + (size_t)node * ab_stride;
+    const int64_t* ipiv = ipiv_all + (size_t)node * n;
+    double* rhs = rhs_all + (size_t)node * n;
+    const double* B0 = b0_all ? b0_all + (size_t)node * n : nullptr;
+    const double* C0 = c0_all ? c0_all + (size_t)node * n : nullptr;
+    const int tid = threadIdx.x;
+    extern __shared__ double sh[];
+    double* x = sh;           // n
+    double* stage = sh + n;   // chunk * (kl+ku+1)
+    const int kd = ku + kl;   // 0-based row of the diagonal
+    const int k = kl + ku;
+    for (int i = tid; i < n; i += kSolveThreads) x[i] = rhs[i];
+    __syncthreads();
+    for (int s = 0; s < nsub; s++) {
+        if (B0) {
+            for (int i = tid; i < n; i += kSolveThreads) {
+                double v = x[i] * B0[i];
+                v += C0[i];
+                x[i] = v;
+            }
+        }
+        __syncthreads();
+        // forward: L with row interchanges (dgbtrs lines 'DO 10 J = 1, N-1')
+        if (kl > 0) {
+            for (int j0 = 0; j0 < n - 1; j0 += chunk) {
+                const int jn = min(chunk, n - 1 - j0);
+                // stage L columns j0..j0+jn-1, rows kd+1..kd+kl
+                for (int e = tid; e < jn * kl; e += kSolveThreads) {
+                    const int cj = e / kl, i = e - cj * kl;
+                    stage[cj * kl + i] = ab[(size_t)(j0 + cj) * ldab + kd + 1 + i];
+                }
+                __syncthreads();
+                for (int cj = 0; cj < jn; cj++) {
+                    const int j = j0 + cj;
+                    const int lm = min(kl, n - 1 - j);
+                    const int l = (int)ipiv[j] - 1;
+                    if (l != j && tid == 0) {
+                        const double t = x[l];
+                        x[l] = x[j];
+                        x[j] = t;
+                    }
+                    __syncthreads();
+                    const double temp = -x[j];
+                    if (temp != 0.0)
+                        for (int i = 1 + tid; i <= lm; i += kSolveThreads) x[j + i] += stage[cj * kl + i - 1] * temp;
+                    __syncthreads();
+                }
+            }
+        }
+        // backward: dtbsv upper, no-transpose, non-unit
+        for (int jhi = n - 1; jhi >= 0; jhi -= chunk) {
+            const int jn = min(chunk, jhi + 1);
+            // stage U columns jhi-jn+1 .. jhi, rows 0..kd
+            for (int e = tid; e < jn * (k + 1); e += kSolveThreads) {
+                const int cj = e / (k + 1), i = e - cj * (k + 1);
+                stage[cj * (k + 1) + i] = ab[(size_t)(jhi - cj) * ldab + i];
+            }
+            __syncthreads();
+            for (int cj = 0; cj < jn; cj++) {
+                const int j = jhi - cj;
+                const double xj = x[j];
+                if (xj != 0.0) {
+                    const double v = xj / stage[cj * (k + 1) + k];
+                    __syncthreads();
+                    if (tid == 0) x[j] = v;
+                    const int lo = max(0, j - k);
+                    for (int i = j - 1 - tid; i >= lo; i -= kSolveThreads)
+                        x[i] = x[i] - v * stage[cj * (k + 1) + k + i - j];
+                }
+                __syncthreads();
+            }
+        }
+    }
+    for (int i = tid; i < n; i += kSolveThreads) rhs[i] = x[i];
+}
+
+namespace {
+
+// rhs (energy-fast) -> psd slice (K-fast) for every node
+__global__ void scatter_psd(const verb_grid g, const double* __restrict__ rhs, double* __restrict__ psd) {
+    const int node = blockIdx.y;
+    const int iR = node % g.nR;
+    if (node_skipped(g, iR)) return;
+    const int nx = g.nV, ny = g.nK, n = nx * ny;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int ix = p / ny, iy = p % ny;
+    psd[(size_t)node * n + p] = rhs[(size_t)node * n + iy * nx + ix];
+}
+
+int solve_chunk(int n, int width) {
+    int dev = 0, max_lds = 0;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    const long avail = (long)max_lds - (long)n * 8;
+    long ch = avail / ((long)width * 8);
+    if (ch > 32) ch = 32;
+    return (int)ch;
+}
+
+}  // namespace
+
+int launch_local(const verb_grid& g, double* psd, const LocalWork& w, bool factor, int* status,
+                 hipStream_t stream, hipEvent_t* marks) {
+    if (g.nV < 3 || g.nK < 3) return VERB_ERR_ARG;
+    const int nx = g.nV, ny = g.nK, n = nx * ny;
+    const int kl = nx + 1, ku = nx + 1, ldab = 2 * kl + ku + 1;
+    const int nodes = g.nP * g.nR;
+    const int num_substeps = max(1, (int)(g.dt / g.sub_dt));
+    const double dt_sub = g.dt / num_substeps;
+    const size_t ab_stride = (size_t)ldab * n;
+    const int chunk = solve_chunk(n, kl + ku + 1);
+    if (chunk < 1) return VERB_ERR_ARG;
+    if (marks) hipEventRecord(marks[0], stream);
+    if (factor) {
+        const size_t count = ab_stride * nodes;
+        hipLaunchKernelGGL(fill_zero, dim3(4096), dim3(256), 0, stream, w.ab, count);
+    }
+    dim3 agrid((n + kAsmThreads - 1) / kAsmThreads, nodes);
+    hipLaunchKernelGGL(local_assemble, agrid, dim3(kAsmThreads), 0, stream, g, psd, w.ab, w.b0, w.c0, w.rhs, dt_sub,
+                       factor ? 1 : 0, status);
+    if (marks) {
+        hipEventRecord(marks[1], stream);
+        hipEventRecord(marks[2], stream);
+    }
+    if (factor) {
+        const size_t lds = sizeof(double) * (size_t)(kl + 1 + kl + ku + 1);
+        hipLaunchKernelGGL(band_factor_kernel, dim3(nodes), dim3(kLuThreads), lds, stream, w.ab, w.ipiv, ab_stride, n,
+                           kl, ku, ldab, status, g.nR, g.include_boundary);
+    }
+    if (marks) {
+        hipEventRecord(marks[3], stream);
+        hipEventRecord(marks[4], stream);
+    }
+    const size_t slds = sizeof(double) * ((size_t)n + (size_t)chunk * (kl + ku + 1));
+    hipFuncSetAttribute((const void*)band_solve_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)slds);
+    hipLaunchKernelGGL(band_solve_kernel, dim3(nodes), dim3(kSolveThreads), slds, stream, w.ab, w.ipiv, ab_stride,
+                       n, kl, ku, ldab, w.b0, w.c0, w.rhs, num_substeps, chunk, g.nR, g.include_boundary);
+    hipLaunchKernelGGL(scatter_psd, agrid, dim3(kAsmThreads), 0, stream, g, w.rhs, psd);
+    if (marks) hipEventRecord(marks[5], stream);
+    return hipGetLastError() == hipSuccess ? VERB_OK : VERB_ERR_HIP;
+}
+
+int launch_dgbtrf(int n, int kl, int ku, double* ab, int ldab, int64_t* ipiv, int* status, hipStream_t stream) {
+    if (n < 1 || kl < 0 || ku < 0 || ldab < 2 * kl + ku + 1) return VERB_ERR_ARG;
+    const size_t lds = sizeof(double) * (size_t)(kl + 1 + kl + ku + 1);
+    hipLaunchKernelGGL(band_factor_kernel, dim3(1), dim3(kLuThreads), lds, stream, ab, ipiv, (size_t)0, n, kl, ku,
+                       ldab, status, 0, 1);
+    return hipGetLastError() == hipSuccess ? VERB_OK : VERB_ERR_HIP;
+}
+
+int launch_dgbtrs(int n, int kl, int ku, const double* ab, int ldab, const int64_t* ipiv, double* b,
+                  hipStream_t stream) {
+    if (n < 1 || kl < 0 || ku < 0 || ldab < 2 * kl + ku + 1) return VERB_ERR_ARG;
+    const int chunk = solve_chunk(n, kl + ku + 1);
+    if (chunk < 1) return VERB_ERR_ARG;
+    const size_t slds = sizeof(double) * ((size_t)n + (size_t)chunk * (kl + ku + 1));
+    hipFuncSetAttribute((const void*)band_solve_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)slds);
+    hipLaunchKernelGGL(band_solve_kernel, dim3(1), dim3(kSolveThreads), slds, stream, ab, ipiv, (size_t)0, n, kl, ku,
+                       ldab, (const double*)nullptr, (const double*)nullptr, b, 1, chunk, 0, 1);
+    return hipGetLastError() == hipSuccess ? VERB_OK : VERB_ERR_HIP;
+}
+
+}  // namespace verb

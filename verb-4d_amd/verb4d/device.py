@@ -1,0 +1,89 @@
+"""Device-resident full-grid state for the batched entries.
+
+Torch supplies HBM allocations only; every operator runs through the C-ABI
+(``verb_step`` and the three batched entries of include/verb4d_hip.h) on the
+context's own HIP stream.
+"""
+import ctypes
+
+import numpy as np
+
+from . import Context, VerbGrid, lib
+
+
+class DeviceGrid:
+    """All fields of a workload in HBM plus the ``verb_grid`` that points at
+    them. ``psd`` is a float64 device tensor [P][R][V][K]."""
+
+    def __init__(self, workload, ctx=None, device="cuda"):
+        import torch
+
+        self.torch = torch
+        self.w = workload
+        self.ctx = ctx or Context()
+        self.device = device
+        self.t = {}
+        self.grid = VerbGrid()
+        workload.fill_struct(self.grid, self._upload)
+        self.psd = torch.from_numpy(np.ascontiguousarray(workload.psd0)).to(device)
+        self.lu_cache = None
+        self.ipiv_cache = None
+        torch.cuda.synchronize()
+
+    def _upload(self, name, arr):
+        torch = self.torch
+        key = id(arr)
+        for k, (aid, tens) in self.t.items():
+            if aid == key:
+                return tens.data_ptr()
+        tens = torch.from_numpy(np.ascontiguousarray(arr, dtype=np.float64)).to(self.device)
+        self.t[name] = (key, tens)
+        return tens.data_ptr()
+
+    def refresh(self, names):
+        """Re-upload fields of the workload that changed on the host (e.g.
+        after ``Workload.set_kp``)."""
+        torch = self.torch
+        for n in names:
+            src = self.w.f[n] if n in self.w.f else self.w.bc[n]
+            _, tens = self.t[n]
+            tens.copy_(torch.from_numpy(np.ascontiguousarray(src)))
+        torch.cuda.synchronize()
+
+    def set_psd(self, arr):
+        self.psd.copy_(self.torch.from_numpy(np.ascontiguousarray(arr, dtype=np.float64)))
+        self.torch.cuda.synchronize()
+
+    def psd_host(self):
+        self.ctx.synchronize()
+        return self.psd.cpu().numpy()
+
+    def alloc_lu_cache(self):
+        torch = self.torch
+        nd = lib().verb_lu_cache_doubles(ctypes.byref(self.grid))
+        ni = lib().verb_lu_cache_ints(ctypes.byref(self.grid))
+        self.lu_cache = torch.empty(nd, dtype=torch.float64, device=self.device)
+        self.ipiv_cache = torch.empty(ni, dtype=torch.int64, device=self.device)
+
+    def step(self, run_convection=True, run_radial=True, run_local=True, use_cache=False, recompute_lu=True):
+        lu = self.lu_cache.data_ptr() if use_cache else None
+        piv = self.ipiv_cache.data_ptr() if use_cache else None
+        rc = lib().verb_step(self.ctx.handle, ctypes.byref(self.grid), self.psd.data_ptr(), int(run_convection),
+                             int(run_radial), int(run_local), lu, piv, 1 if recompute_lu else 0)
+        self.ctx.check(rc)
+
+    def convection(self):
+        self.ctx.check(lib().verb_convection_batched(self.ctx.handle, ctypes.byref(self.grid), self.psd.data_ptr()))
+
+    def radial(self):
+        self.ctx.check(lib().verb_radial_diffusion_batched(self.ctx.handle, ctypes.byref(self.grid),
+                                                           self.psd.data_ptr()))
+
+    def local(self, use_cache=False, recompute_lu=True):
+        lu = self.lu_cache.data_ptr() if use_cache else None
+        piv = self.ipiv_cache.data_ptr() if use_cache else None
+        self.ctx.check(lib().verb_local_diffusion_batched(self.ctx.handle, ctypes.byref(self.grid),
+                                                          self.psd.data_ptr(), lu, piv, 1 if recompute_lu else 0))
+
+    def check_status(self):
+        return self.ctx.check_status()
