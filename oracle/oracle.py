@@ -45,7 +45,14 @@ def build():
 
 
 _oracle = None
-_ref = None
+_ref = {}
+# reference harness builds: the reference alone, and the reference with the
+# product's drop-ins linked in (built only where the product library exists)
+REF_VARIANTS = {
+    "ref": REF_LIB,
+    "hipops": os.path.join(HERE, "_ref", "libverbref_hipops.so"),
+    "hiplapack": os.path.join(HERE, "_ref", "libverbref_hiplapack.so"),
+}
 
 
 def oracle_lib():
@@ -72,16 +79,16 @@ def oracle_lib():
     return _oracle
 
 
-def ref_available():
-    return os.path.exists(REF_LIB)
+def ref_available(variant="ref"):
+    return os.path.exists(REF_VARIANTS[variant])
 
 
-def ref_lib():
-    global _ref
-    if _ref is None:
-        if not ref_available():
-            raise RuntimeError("reference harness not built (%s)" % REF_LIB)
-        h = ctypes.CDLL(REF_LIB)
+def ref_lib(variant="ref"):
+    if variant not in _ref:
+        path = REF_VARIANTS[variant]
+        if not os.path.exists(path):
+            raise RuntimeError("reference harness not built (%s)" % path)
+        h = ctypes.CDLL(path)
         h.ref_diffusion_2d.argtypes = [_dp, _dp, _dp, _I, _I, _dp, _dp, _dp, _dp, _I, _I, _I, _I] + [_dp] * 7 + [_D, _D]
         h.ref_assemble_2d.argtypes = [_I, _I, _dp, _dp, _dp, _dp, _dp, _dp, _I, _I, _I, _I] + [_dp] * 7 + [
             _D, _dp, _dp, _dp]
@@ -93,8 +100,8 @@ def ref_lib():
                                       ctypes.c_long, ctypes.c_long, ctypes.c_long, ctypes.c_long]
         h.ref_grid_struct_size.restype = ctypes.c_size_t
         assert h.ref_grid_struct_size() == ctypes.sizeof(Grid)
-        _ref = h
-    return _ref
+        _ref[variant] = h
+    return _ref[variant]
 
 
 def _c(a):
@@ -121,7 +128,7 @@ def diffusion_2d(psd, x, y, nx, ny, xl, xu, yl, yu, types, Dxx, Dyy, Dxy, Dyx, G
                                            _p(a[5]), *types, *[_p(v) for v in a[6:]], dt, sub_dt,
                                            ctypes.cast(None, _dp), ctypes.cast(None, _i64p), 1)
     else:
-        st = ref_lib().ref_diffusion_2d(_p(out), _p(a[0]), _p(a[1]), nx, ny, _p(a[2]), _p(a[3]), _p(a[4]),
+        st = ref_lib(lib).ref_diffusion_2d(_p(out), _p(a[0]), _p(a[1]), nx, ny, _p(a[2]), _p(a[3]), _p(a[4]),
                                         _p(a[5]), *types, *[_p(v) for v in a[6:]], dt, sub_dt)
     if st != 0:
         raise RuntimeError("diffusion_2d status %d" % st)
@@ -134,7 +141,7 @@ def assemble_2d(x, y, nx, ny, xl, xu, yl, yu, types, Dxx, Dyy, Dxy, Dyx, G, S, L
     B0 = np.zeros(n)
     C0 = np.zeros(n)
     a = _local_args(x, y, nx, ny, xl, xu, yl, yu, Dxx, Dyy, Dxy, Dyx, G, S, L)
-    f = oracle_lib().orc_assemble_2d if lib == "oracle" else ref_lib().ref_assemble_2d
+    f = oracle_lib().orc_assemble_2d if lib == "oracle" else ref_lib(lib).ref_assemble_2d
     st = f(nx, ny, _p(a[0]), _p(a[1]), _p(a[2]), _p(a[3]), _p(a[4]), _p(a[5]), *types, *[_p(v) for v in a[6:]], dt,
            _p(diag), _p(B0), _p(C0))
     if st != 0:
@@ -173,7 +180,7 @@ def dgbtrs(lu, kl, ku, b, ipiv):
 def tridag(a, b, c, r, lib="oracle"):
     a, b, c, r = (_c(v) for v in (a, b, c, r))
     u = np.zeros_like(r)
-    f = oracle_lib().orc_tridag if lib == "oracle" else ref_lib().ref_tridag
+    f = oracle_lib().orc_tridag if lib == "oracle" else ref_lib(lib).ref_tridag
     f(_p(a), _p(b), _p(c), _p(r), _p(u), len(r))
     return u
 
@@ -185,7 +192,7 @@ def diffusion_1d(psd, x, n, lbc, ubc, lt, ut, D, G, S, L, dt, lib="oracle"):
         st = oracle_lib().orc_diffusion_1d(_p(out), _p(a[0]), n, lbc, ubc, lt, ut, _p(a[1]), _p(a[2]), _p(a[3]),
                                            _p(a[4]), dt)
     else:
-        st = ref_lib().ref_diffusion_1d(_p(out), _p(a[0]), n, lbc, ubc, lt, ut, _p(a[1]), _p(a[2]), _p(a[3]),
+        st = ref_lib(lib).ref_diffusion_1d(_p(out), _p(a[0]), n, lbc, ubc, lt, ut, _p(a[1]), _p(a[2]), _p(a[3]),
                                         _p(a[4]), dt)
     if st != 0:
         raise RuntimeError("diffusion_1d status %d" % st)
@@ -198,7 +205,7 @@ def quickest6(psd, x, n, lbc, ubc, lt, ut, ux, dt, lib="oracle"):
     if lib == "oracle":
         st = oracle_lib().orc_quickest6(_p(out), _p(xx), n, lbc, ubc, lt, ut, _p(uu), dt)
     else:
-        st = ref_lib().ref_quickest6(_p(out), _p(xx), n, lbc, ubc, lt, ut, _p(uu), dt)
+        st = ref_lib(lib).ref_quickest6(_p(out), _p(xx), n, lbc, ubc, lt, ut, _p(uu), dt)
     if st != 0:
         raise RuntimeError("quickest6 status %d" % st)
     return out
@@ -207,7 +214,7 @@ def quickest6(psd, x, n, lbc, ubc, lt, ut, ux, dt, lib="oracle"):
 def convection_2d(psd, P, R, nP, nR, Pl, Pu, Rl, Ru, types, VP, VR, S, L, G, dt, lib="oracle"):
     out = _c(psd).copy()
     a = [_c(v) for v in (P, R, Pl, Pu, Rl, Ru, VP, VR, S, L, G)]
-    f = oracle_lib().orc_convection_2d if lib == "oracle" else ref_lib().ref_convection_2d
+    f = oracle_lib().orc_convection_2d if lib == "oracle" else ref_lib(lib).ref_convection_2d
     st = f(_p(out), _p(a[0]), _p(a[1]), nP, nR, _p(a[2]), _p(a[3]), _p(a[4]), _p(a[5]), *types,
            *[_p(v) for v in a[6:]], dt)
     if st != 0:
@@ -230,7 +237,7 @@ def step(w, psd, nthreads=0, run_convection=1, run_radial=1, run_local=1, lib="o
     if lib == "oracle":
         st = oracle_lib().orc_step(ctypes.byref(g), _p(out), nthreads, run_convection, run_radial, run_local)
     else:
-        st = ref_lib().ref_step_sample(ctypes.byref(g), _p(out), nthreads, run_convection, run_radial, run_local,
+        st = ref_lib(lib).ref_step_sample(ctypes.byref(g), _p(out), nthreads, run_convection, run_radial, run_local,
                                        0, -1, 0, -1, 0, -1)
     if st != 0:
         raise RuntimeError("step status %d" % st)

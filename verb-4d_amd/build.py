@@ -22,6 +22,7 @@ SOURCES = [
     ("radial.hip", ["-ffp-contract=off"]),
     ("convection.hip", ["-ffp-contract=off"]),
     ("local.hip", ["-ffp-contract=off"]),
+    ("band_lu.hip", []),
     ("api.hip", []),
 ]
 

@@ -112,12 +112,15 @@ LocalBuffers local_buffers(verb_ctx* c, const verb_grid* g, double* lu_cache, in
     lb.w.b0 = (double*)scratch(c, base_slot + 2, n * nodes * sizeof(double));
     lb.w.c0 = (double*)scratch(c, base_slot + 3, n * nodes * sizeof(double));
     lb.w.rhs = (double*)scratch(c, base_slot + 4, n * nodes * sizeof(double));
-    if (!lb.w.ab || !lb.w.ipiv || !lb.w.b0 || !lb.w.c0 || !lb.w.rhs)
+    lb.w.diag = (double*)scratch(c, base_slot + 5, 9 * n * nodes * sizeof(double));
+    lb.w.y = (double*)scratch(c, base_slot + 6, n * nodes * sizeof(double));
+    lb.w.uext = (int*)scratch(c, base_slot + 7, n * nodes * sizeof(int));
+    if (!lb.w.ab || !lb.w.ipiv || !lb.w.b0 || !lb.w.c0 || !lb.w.rhs || !lb.w.diag || !lb.w.y || !lb.w.uext)
         lb.rc = set_err(c, VERB_ERR_NOMEM, "device allocation failed");
     return lb;
 }
 
-constexpr int kSlotLocal = 40;  // scratch slots 40..44 for the local workspace
+constexpr int kSlotLocal = 40;  // scratch slots 40..47 for the local workspace
 
 hipEvent_t new_event() {
     hipEvent_t e = nullptr;
@@ -283,7 +286,7 @@ int verb_local_diffusion_batched(verb_ctx* c, const verb_grid* g, double* psd, d
         for (auto& e : ev) e = new_event();
         marks = ev;
     }
-    int rc = verb::launch_local(*g, psd, lb.w, factor, c->status, c->stream, marks);
+    int rc = verb::launch_local(*g, psd, lb.w, factor, lu_cache != nullptr, c->status, c->stream, marks);
     if (marks) {
         c->marks.push_back({VERB_PHASE_LOCAL_ASSEMBLE, ev[0], ev[1]});
         c->marks.push_back({VERB_PHASE_LOCAL_FACTOR, ev[2], ev[3]});
@@ -357,12 +360,12 @@ int verb_diffusion_2d(verb_ctx* c, double* psd, const double* x, const double* y
             HIPCHK(c, hipMemcpyAsync(dpiv, ipiv_cache, n * sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
         }
     }
-    LocalBuffers lb = local_buffers(c, &g, dab, dpiv, 20);
+    LocalBuffers lb = local_buffers(c, &g, dab, dpiv, 20);  // slots 20..27
     if (lb.rc) return lb.rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if ((rc = reset_status(c))) return rc;
     const bool factor = (lu_cache == nullptr) || recompute_lu;
-    rc = verb::launch_local(g, dpsd, lb.w, factor, c->status, c->stream);
+    rc = verb::launch_local(g, dpsd, lb.w, factor, lu_cache != nullptr, c->status, c->stream);
     if (rc) return set_err(c, rc, "local diffusion launch rejected");
     rc = finish(c);
     reset_status(c);

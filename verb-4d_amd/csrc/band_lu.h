@@ -1,0 +1,19 @@
+// Arguments of the register-window band LU (band_lu.hip).
+#pragma once
+#include <cstdint>
+
+namespace verb {
+
+struct LuArgs {
+    const double* diag;  // [nodes][9][n] assembled operator, energy-fast rows
+    const double* rhs;   // [nodes][n] B0*psd + C0 (energy-fast), or null
+    double* y;           // [nodes][n] forward-eliminated right-hand side, or null
+    double* ab;          // [nodes][ldab*n] LAPACK band factors
+    int64_t* ipiv;       // [nodes][n] 1-based pivots
+    int* uext;           // [nodes][n] last nonzero column of each U row, or null
+    int n, nx, write_L;
+    int* status;
+    int nR, include_boundary;  // node skip rule (nR = 0: factor every node)
+};
+
+}  // namespace verb

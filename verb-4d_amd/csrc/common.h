@@ -32,10 +32,22 @@ struct LocalWork {
     double* b0;       // per node n: diagonal of B (energy-fast order)
     double* c0;       // per node n: diagonal of C
     double* rhs;      // per node n: right-hand side / solution (energy-fast order)
+    double* diag;     // per node 9n: the assembled operator (register-window LU path)
+    double* y;        // per node n: forward-eliminated right-hand side
+    int* uext;        // per node n: last nonzero column of each U row
 };
+
+struct LuArgs;
+int launch_band_lu(const LuArgs& a, int nodes, hipStream_t stream);
+int launch_backsolve(const double* ab, const double* y, const int* uext, double* x, int n, int kl, int ku, int ldab,
+                     int nodes, int nR, int include_boundary, hipStream_t stream);
+// 0 = automatic (register-window LU where an instantiation covers kl),
+// 1 = force the one-column-per-step reference-order kernel.
+extern int local_lu_variant;
 // marks (nullable, 6 events): begin/end of the assembly, of the band factor
 // and of the solve (+ scatter), recorded on `stream`.
-int launch_local(const verb_grid& g, double* psd, const LocalWork& w, bool factor, int* status,
+// cached: the factors live in a caller's cache (LAPACK layout incl. L).
+int launch_local(const verb_grid& g, double* psd, const LocalWork& w, bool factor, bool cached, int* status,
                  hipStream_t stream, hipEvent_t* marks = nullptr);
 
 // Single-system LAPACK band routines on device buffers (one system).

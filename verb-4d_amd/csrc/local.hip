@@ -18,6 +18,7 @@
 // The unknown vector is energy-fast (in = iK*nV + iV, src/Matrix.cpp:6479),
 // so kl = ku = nV+1 as in the reference.
 #include "common.h"
+#include "band_lu.h"
 
 namespace verb {
 namespace {
@@ -121,7 +122,8 @@ __device__ inline bool node_skipped(const verb_grid& g, int iR) {
 __global__ __launch_bounds__(kAsmThreads) void local_assemble(const verb_grid g, const double* __restrict__ psd,
                                                              double* __restrict__ ab, double* __restrict__ b0,
                                                              double* __restrict__ c0, double* __restrict__ rhs,
-                                                             double dt_sub, int write_band, int* status) {
+                                                             double* __restrict__ diag, double dt_sub,
+                                                             int write_band, int premul, int* status) {
     const int node = blockIdx.y;
     const int iP = node / g.nR, iR = node % g.nR;
     if (node_skipped(g, iR)) return;
@@ -136,7 +138,19 @@ __global__ __launch_bounds__(kAsmThreads) void local_assemble(const verb_grid g,
     const size_t nn = (size_t)node * n;
     b0[nn + in] = B0;
     c0[nn + in] = C0;
-    rhs[nn + in] = psd[nn + p];
+    if (premul) {
+        // first sub-step's right-hand side: rhs = rhs*B0 + C0 (MatrixSolver.cpp:318-319)
+        double v = psd[nn + p] * B0;
+        v += C0;
+        rhs[nn + in] = v;
+    } else {
+        rhs[nn + in] = psd[nn + p];
+    }
+    if (diag) {
+        double* dn = diag + (size_t)node * 9 * n;
+#pragma unroll
+        for (int sl = 0; sl < 9; sl++) dn[(size_t)sl * n + in] = A[sl];
+    }
     if (write_band) {
         const int kl = nx + 1, ku = nx + 1, ldab = 2 * kl + ku + 1;
         double* abn = ab + (size_t)node * ldab * n;
@@ -371,7 +385,7 @@ int solve_chunk(int n, int width) {
 
 }  // namespace
 
-int launch_local(const verb_grid& g, double* psd, const LocalWork& w, bool factor, int* status,
+int launch_local(const verb_grid& g, double* psd, const LocalWork& w, bool factor, bool cached, int* status,
                  hipStream_t stream, hipEvent_t* marks) {
     if (g.nV < 3 || g.nK < 3) return VERB_ERR_ARG;
     const int nx = g.nV, ny = g.nK, n = nx * ny;
@@ -382,19 +396,54 @@ int launch_local(const verb_grid& g, double* psd, const LocalWork& w, bool facto
     const size_t ab_stride = (size_t)ldab * n;
     const int chunk = solve_chunk(n, kl + ku + 1);
     if (chunk < 1) return VERB_ERR_ARG;
+    dim3 agrid((n + kAsmThreads - 1) / kAsmThreads, nodes);
+    const bool window = local_lu_variant == 0 && kl <= 112 && w.diag && w.y && w.uext;
+    if (window && factor && num_substeps == 1 && !cached) {
+        // plain path: factor with the forward elimination fused, then U x = y
+        if (marks) hipEventRecord(marks[0], stream);
+        hipLaunchKernelGGL(local_assemble, agrid, dim3(kAsmThreads), 0, stream, g, psd, w.ab, w.b0, w.c0, w.rhs,
+                           w.diag, dt_sub, 0, 1, status);
+        if (marks) {
+            hipEventRecord(marks[1], stream);
+            hipEventRecord(marks[2], stream);
+        }
+        LuArgs la;
+        la.diag = w.diag; la.rhs = w.rhs; la.y = w.y; la.ab = w.ab; la.ipiv = w.ipiv; la.uext = w.uext;
+        la.n = n; la.nx = nx; la.write_L = 0; la.status = status; la.nR = g.nR;
+        la.include_boundary = g.include_boundary;
+        int rc = launch_band_lu(la, nodes, stream);
+        if (rc) return rc;
+        if (marks) {
+            hipEventRecord(marks[3], stream);
+            hipEventRecord(marks[4], stream);
+        }
+        rc = launch_backsolve(w.ab, w.y, w.uext, w.rhs, n, kl, ku, ldab, nodes, g.nR, g.include_boundary, stream);
+        if (rc) return rc;
+        hipLaunchKernelGGL(scatter_psd, agrid, dim3(kAsmThreads), 0, stream, g, w.rhs, psd);
+        if (marks) hipEventRecord(marks[5], stream);
+        return hipGetLastError() == hipSuccess ? VERB_OK : VERB_ERR_HIP;
+    }
     if (marks) hipEventRecord(marks[0], stream);
     if (factor) {
         const size_t count = ab_stride * nodes;
         hipLaunchKernelGGL(fill_zero, dim3(4096), dim3(256), 0, stream, w.ab, count);
     }
-    dim3 agrid((n + kAsmThreads - 1) / kAsmThreads, nodes);
-    hipLaunchKernelGGL(local_assemble, agrid, dim3(kAsmThreads), 0, stream, g, psd, w.ab, w.b0, w.c0, w.rhs, dt_sub,
-                       factor ? 1 : 0, status);
+    hipLaunchKernelGGL(local_assemble, agrid, dim3(kAsmThreads), 0, stream, g, psd, w.ab, w.b0, w.c0, w.rhs,
+                       (window && factor) ? w.diag : (double*)nullptr, dt_sub, (factor && !window) ? 1 : 0, 0,
+                       status);
     if (marks) {
         hipEventRecord(marks[1], stream);
         hipEventRecord(marks[2], stream);
     }
-    if (factor) {
+    if (factor && window) {
+        // cached factors / several sub-steps: keep L (dgbtf2 order) as well
+        LuArgs la;
+        la.diag = w.diag; la.rhs = nullptr; la.y = nullptr; la.ab = w.ab; la.ipiv = w.ipiv; la.uext = w.uext;
+        la.n = n; la.nx = nx; la.write_L = 1; la.status = status; la.nR = g.nR;
+        la.include_boundary = g.include_boundary;
+        int rc = launch_band_lu(la, nodes, stream);
+        if (rc) return rc;
+    } else if (factor) {
         const size_t lds = sizeof(double) * (size_t)(kl + 1 + kl + ku + 1);
         hipLaunchKernelGGL(band_factor_kernel, dim3(nodes), dim3(kLuThreads), lds, stream, w.ab, w.ipiv, ab_stride, n,
                            kl, ku, ldab, status, g.nR, g.include_boundary);
