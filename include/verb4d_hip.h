@@ -159,6 +159,19 @@ int verb_local_diffusion_batched(verb_ctx* ctx, const verb_grid* g, double* psd,
 size_t verb_lu_cache_doubles(const verb_grid* g);
 size_t verb_lu_cache_ints(const verb_grid* g);
 
+/* Local diffusion with an uneven active pitch-angle extent per node, as the
+ * reference expresses it by calling Diffusion_2D with a trimmed y_size and
+ * sliced arrays (src/Diffusion_2D.h:100-110). k0[node] (host array of
+ * nP*nR ints, node = iP*nR + iR) is the first active K column: the node
+ * solves the system of the slice [V][k0:K], whose lower pitch-angle row
+ * takes the K-lower boundary condition; columns below k0 are left
+ * untouched. Nodes are dispatched largest first. No reference counterpart
+ * as a batched entry. */
+int verb_local_diffusion_uneven(verb_ctx* ctx, const verb_grid* g, double* psd, const int* k0, double* lu_cache,
+                                int64_t* ipiv_cache, int recompute_lu);
+int verb_step_uneven(verb_ctx* ctx, const verb_grid* g, double* psd, const int* k0, int run_convection,
+                     int run_radial, int run_local);
+
 /* One full operator-split step: convection, radial, local
  * (src/VERB4D_Solver.cpp:851-1216), each enabled by its run_* flag. */
 int verb_step(verb_ctx* ctx, const verb_grid* g, double* psd, int run_convection, int run_radial, int run_local,

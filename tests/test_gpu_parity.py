@@ -245,6 +245,41 @@ def test_dropins_inside_reference_time_loop(variant, threads, golden_dir):
     assert node_rel_err(out, d["out"]) < 1e-12
 
 
+def uneven_k0(w, seed):
+    """First active pitch-angle column per node: a loss-cone part growing
+    inward with R plus a seeded MLT-dependent band of up to +-3 cells."""
+    rng = np.random.default_rng(seed)
+    base = np.round(np.linspace(0.55, 0.0, w.nR) * (w.nK - 3)).astype(int)
+    k0 = base[None, :] + rng.integers(-3, 4, size=(w.nP, w.nR))
+    return np.clip(k0, 0, w.nK - 3).astype(np.int32)
+
+
+@pytest.mark.parametrize("cfg", ["config0", "wide"])
+def test_uneven_extents_match_trimmed_reference_calls(ctx, cfg):
+    """Each node's system is the reference's Diffusion_2D on the slice
+    [V][k0:K] (trimmed y_size); columns below k0 stay untouched."""
+    if cfg == "config0":
+        w = W.config0(seed=41, rich=True)
+    else:
+        w = W.Workload(nP=2, nR=4, nV=101, nK=91, seed=42, kp=2.0)
+    k0 = uneven_k0(w, 5)
+    dg = _device_grid(w, ctx)
+    dg.local_uneven(k0)
+    dg.check_status()
+    got = dg.psd_host()
+    for iP in range(w.nP):
+        for iR in range(w.nR):
+            k = int(k0[iP, iR])
+            o = (iP, iR)
+            sl = (iP, iR, slice(None), slice(k, None))
+            ref = O.diffusion_2d(w.psd0[sl], w.f["Vg"][sl], w.f["Kg"][sl], w.nV, w.nK - k, w.bc["Vl"][o][k:],
+                                 w.bc["Vu"][o][k:], w.bc["Kl"][o], w.bc["Ku"][o], (1, 1, 1, 2), w.f["DVV"][sl],
+                                 w.f["DKK"][sl], w.f["DVK"][sl], w.f["DVK"][sl], w.f["G_local"][sl],
+                                 w.f["sources"][sl], w.f["losses_local"][sl], w.dt, w.sub_dt)
+            assert rel_err(got[sl], ref) < 1e-12, (iP, iR, k)
+            assert bit_equal(got[iP, iR, :, :k], w.psd0[iP, iR, :, :k])
+
+
 def test_batched_production_grid_one_step(ctx):
     """29x24x101x91 at Kp = 6: one full step against the oracle."""
     say("production step")

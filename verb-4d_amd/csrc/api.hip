@@ -295,6 +295,58 @@ int verb_local_diffusion_batched(verb_ctx* c, const verb_grid* g, double* psd, d
     return rc ? set_err(c, rc, "local diffusion launch rejected") : VERB_OK;
 }
 
+int verb_local_diffusion_uneven(verb_ctx* c, const verb_grid* g, double* psd, const int* k0, double* lu_cache,
+                                int64_t* ipiv_cache, int recompute_lu) {
+    if (!c || !grid_ok(g) || !psd || !k0) return VERB_ERR_ARG;
+    if ((lu_cache == nullptr) != (ipiv_cache == nullptr)) return VERB_ERR_ARG;
+    if (!(g->sub_dt > 0)) return VERB_ERR_ARG;
+    const int nodes = g->nP * g->nR;
+    std::vector<int> hk(k0, k0 + nodes), hn(nodes), ord(nodes);
+    for (int i = 0; i < nodes; i++) {
+        if (hk[i] < 0 || g->nK - hk[i] < 3) return set_err(c, VERB_ERR_ARG, "active pitch-angle extent below 3");
+        hn[i] = g->nV * (g->nK - hk[i]);
+        ord[i] = i;
+    }
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return hn[a] > hn[b]; });
+    std::lock_guard<std::mutex> lk(c->mu);
+    int* dk = (int*)scratch(c, 50, nodes * sizeof(int));
+    int* dn = (int*)scratch(c, 51, nodes * sizeof(int));
+    int* dord = (int*)scratch(c, 52, nodes * sizeof(int));
+    if (!dk || !dn || !dord) return set_err(c, VERB_ERR_NOMEM, "device allocation failed");
+    HIPCHK(c, hipMemcpyAsync(dk, hk.data(), nodes * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(dn, hn.data(), nodes * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(dord, ord.data(), nodes * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    // the host vectors die at return: wait for the copies
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    LocalBuffers lb = local_buffers(c, g, lu_cache, ipiv_cache, kSlotLocal);
+    if (lb.rc) return lb.rc;
+    const bool factor = (lu_cache == nullptr) || recompute_lu;
+    verb::NodeSizes sz{dk, dn, dord};
+    hipEvent_t ev[6];
+    hipEvent_t* marks = nullptr;
+    if (c->timing) {
+        for (auto& e : ev) e = new_event();
+        marks = ev;
+    }
+    int rc = verb::launch_local(*g, psd, lb.w, factor, lu_cache != nullptr, c->status, c->stream, marks, &sz);
+    if (marks) {
+        c->marks.push_back({VERB_PHASE_LOCAL_ASSEMBLE, ev[0], ev[1]});
+        c->marks.push_back({VERB_PHASE_LOCAL_FACTOR, ev[2], ev[3]});
+        c->marks.push_back({VERB_PHASE_LOCAL_SOLVE, ev[4], ev[5]});
+    }
+    return rc ? set_err(c, rc, "uneven local diffusion launch rejected") : VERB_OK;
+}
+
+int verb_step_uneven(verb_ctx* c, const verb_grid* g, double* psd, const int* k0, int run_convection, int run_radial,
+                     int run_local) {
+    int rc = VERB_OK;
+    if (run_convection && (3 < g->nP || 3 < g->nR)) rc = verb_convection_batched(c, g, psd);
+    if (!rc && run_radial && g->nR >= 3) rc = verb_radial_diffusion_batched(c, g, psd);
+    if (!rc && run_local && g->nV >= 3 && g->nK >= 3)
+        rc = verb_local_diffusion_uneven(c, g, psd, k0, nullptr, nullptr, 1);
+    return rc;
+}
+
 int verb_step(verb_ctx* c, const verb_grid* g, double* psd, int run_convection, int run_radial, int run_local,
               double* lu_cache, int64_t* ipiv_cache, int recompute_lu) {
     int rc = VERB_OK;

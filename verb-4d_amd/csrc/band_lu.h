@@ -14,6 +14,8 @@ struct LuArgs {
     int n, nx, write_L;
     int* status;
     int nR, include_boundary;  // node skip rule (nR = 0: factor every node)
+    const int* nn;             // per-node unknowns (null: n for every node); buffers keep stride n
+    const int* order;          // dispatch order of the nodes (null: identity)
 };
 
 }  // namespace verb

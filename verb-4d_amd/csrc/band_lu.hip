@@ -54,16 +54,35 @@ __device__ inline double a_orig(const double* dg, int n, int nx, int r, int c) {
     return dg[(size_t)((dy + 1) * 3 + dx + 1) * n + r];
 }
 
+// Accumulator layout of v_mfma_f64_16x16x4f64: lane j + 16*g, element r
+// holds row 4*r + g, column j of the 16x16 tile (rows interleave across the
+// four 16-lane groups).
+__device__ inline int crow(int g, int r) { return 4 * r + g; }
+
 __device__ inline void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// DPP move of a double within 16-lane rows (ctrl: a row-local DPP pattern).
+template <int CTRL>
+__device__ inline double dpp_row(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Wavefront maximum: quad_perm [1,0,3,2], quad_perm [2,3,0,1],
+// row_half_mirror and row_mirror leave every lane of a 16-lane row with the
+// row maximum; the four row maxima are then combined.
 __device__ inline double wave_max(double v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v = fmax(v, __shfl_xor(v, off));
-    return v;
+    v = fmax(v, dpp_row<0xB1>(v));
+    v = fmax(v, dpp_row<0x4E>(v));
+    v = fmax(v, dpp_row<0x141>(v));
+    v = fmax(v, dpp_row<0x140>(v));
+    const double r0 = __shfl(v, 0), r1 = __shfl(v, 16), r2 = __shfl(v, 32), r3 = __shfl(v, 48);
+    return fmax(fmax(r0, r1), fmax(r2, r3));
 }
 
 template <int RB, int CB>
@@ -92,16 +111,17 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgs a) {
     extern __shared__ double lds_raw[];
     L& S = *reinterpret_cast<L*>(lds_raw);
 
-    const int node = blockIdx.x;
+    const int node = a.order ? a.order[blockIdx.x] : blockIdx.x;
     if (a.nR > 0 && !a.include_boundary && (node % a.nR == 0 || node % a.nR == a.nR - 1)) return;
-    const int n = a.n, nx = a.nx;
+    const int stride = a.n;
+    const int n = a.nn ? a.nn[node] : a.n, nx = a.nx;
     const int kl = nx + 1, ku = nx + 1, kv = kl + ku, ldab = 2 * kl + ku + 1;
-    const double* dg = a.diag + (size_t)node * 9 * n;
-    const double* rhs = a.rhs ? a.rhs + (size_t)node * n : nullptr;
-    double* yv = a.y ? a.y + (size_t)node * n : nullptr;
-    double* ab = a.ab + (size_t)node * ldab * n;
-    int64_t* ipiv = a.ipiv + (size_t)node * n;
-    int* uext = a.uext ? a.uext + (size_t)node * n : nullptr;
+    const double* dg = a.diag + (size_t)node * 9 * stride;
+    const double* rhs = a.rhs ? a.rhs + (size_t)node * stride : nullptr;
+    double* yv = a.y ? a.y + (size_t)node * stride : nullptr;
+    double* ab = a.ab + (size_t)node * ldab * stride;
+    int64_t* ipiv = a.ipiv + (size_t)node * stride;
+    int* uext = a.uext ? a.uext + (size_t)node * stride : nullptr;
 
     const int tid = threadIdx.x;
     const int w = tid >> 6, l = tid & 63, g = l >> 4, jj = l & 15;
@@ -111,7 +131,7 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgs a) {
 #pragma unroll
     for (int s = 0; s < CB; s++)
 #pragma unroll
-        for (int r = 0; r < 4; r++) acc[s][r] = a_orig(dg, n, nx, NB * w + 4 * g + r, NB * s + jj);
+        for (int r = 0; r < 4; r++) acc[s][r] = a_orig(dg, n, nx, NB * w + crow(g, r), NB * s + jj);
     for (int i = tid; i < PR; i += RB * 64) S.rw[i] = (rhs && i < n) ? rhs[i] : 0.0;
     if (tid == 0) {
         S.ju = 0;
@@ -131,7 +151,7 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgs a) {
         for (int s = 0; s < CB; s++)
             if (s == s0) {
 #pragma unroll
-                for (int r = 0; r < 4; r++) S.pnl[(NB * lb + 4 * g + r) * NB + jj] = acc[s][r];
+                for (int r = 0; r < 4; r++) S.pnl[(NB * lb + crow(g, r)) * NB + jj] = acc[s][r];
             }
         __syncthreads();
 
@@ -236,8 +256,8 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgs a) {
         // 3a. rows touched by the interchanges -> LDS (columns right of the panel)
         for (int q = 0; q < ns; q++) {
             const int row = S.srow[q];
-            if (row / NB == lb && ((row % NB) >> 2) == g) {
-                const int r = row & 3;
+            if (row / NB == lb && (row & 3) == g) {
+                const int r = (row & 15) >> 2;
 #pragma unroll
                 for (int s = 0; s < CB; s++) {
                     if (s == s0) continue;
@@ -271,8 +291,8 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgs a) {
         for (int q = NB; q < ns; q++) {
             const int row = S.srow[q];
             if (S.ssrc[q] == q) continue;
-            if (row / NB == lb && ((row % NB) >> 2) == g) {
-                const int r = row & 3;
+            if (row / NB == lb && (row & 3) == g) {
+                const int r = (row & 15) >> 2;
                 const int src = S.ssrc[q];
 #pragma unroll
                 for (int s = 0; s < CB; s++) {
@@ -314,7 +334,7 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgs a) {
                 if (s == s0) {
 #pragma unroll
                     for (int r = 0; r < 4; r++)
-                        acc[s][r] = a_orig(dg, n, nx, NB * rb + 4 * g + r, NB * (p + CB) + jj);
+                        acc[s][r] = a_orig(dg, n, nx, NB * rb + crow(g, r), NB * (p + CB) + jj);
                 }
         } else {
             // U rows j0..j0+nc-1: columns r..ext(r); beyond ext(r) the row is
@@ -355,7 +375,7 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgs a) {
             for (int s = 0; s < CB; s++) {
                 const int cb = (p + 1) + ((s - (p + 1) % CB) + CB) % CB;
 #pragma unroll
-                for (int r = 0; r < 4; r++) acc[s][r] = a_orig(dg, n, nx, NB * rb + 4 * g + r, NB * cb + jj);
+                for (int r = 0; r < 4; r++) acc[s][r] = a_orig(dg, n, nx, NB * rb + crow(g, r), NB * cb + jj);
             }
             for (int c = l; c < NB; c += 64) {
                 const int i = NB * rb + c;
@@ -380,15 +400,19 @@ __global__ __launch_bounds__(kBsThreads) void band_backsolve(const double* __res
                                                              const double* __restrict__ y_all,
                                                              const int* __restrict__ uext_all,
                                                              double* __restrict__ x_all, int n, int kl, int ku,
-                                                             int ldab, int nR, int include_boundary) {
-    const int node = blockIdx.x;
+                                                             int ldab, int nR, int include_boundary,
+                                                             const int* __restrict__ nn,
+                                                             const int* __restrict__ order) {
+    const int node = order ? order[blockIdx.x] : blockIdx.x;
     if (nR > 0 && !include_boundary && (node % nR == 0 || node % nR == nR - 1)) return;
+    const int stride = n;
+    if (nn) n = nn[node];
     const int kv = kl + ku;
     const int WS = kv + NB;  // staged columns per block (relative to the block start)
-    const double* ab = ab_all + (size_t)node * ldab * n;
-    const double* yv = y_all + (size_t)node * n;
-    const int* ext = uext_all + (size_t)node * n;
-    double* xo = x_all + (size_t)node * n;
+    const double* ab = ab_all + (size_t)node * ldab * stride;
+    const double* yv = y_all + (size_t)node * stride;
+    const int* ext = uext_all + (size_t)node * stride;
+    double* xo = x_all + (size_t)node * stride;
     extern __shared__ double sh[];
     double* x = sh;                       // n
     double* stg = sh + n;                 // 2 x [NB][WS]
@@ -460,7 +484,7 @@ __global__ __launch_bounds__(kBsThreads) void band_backsolve(const double* __res
 }
 
 int launch_backsolve(const double* ab, const double* y, const int* uext, double* x, int n, int kl, int ku, int ldab,
-                     int nodes, int nR, int include_boundary, hipStream_t stream) {
+                     int nodes, int nR, int include_boundary, hipStream_t stream, const int* nn, const int* order) {
     const int WS = kl + ku + NB;
     const size_t lds = sizeof(double) * ((size_t)n + 2 * NB * WS + NB * 16);
     int dev = 0, max_lds = 0;
@@ -469,7 +493,7 @@ int launch_backsolve(const double* ab, const double* y, const int* uext, double*
     if (lds + 256 > (size_t)max_lds) return VERB_ERR_ARG;
     hipFuncSetAttribute((const void*)band_backsolve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(band_backsolve, dim3(nodes), dim3(kBsThreads), lds, stream, ab, y, uext, x, n, kl, ku, ldab, nR,
-                       include_boundary);
+                       include_boundary, nn, order);
     return hipGetLastError() == hipSuccess ? VERB_OK : VERB_ERR_HIP;
 }
 

@@ -40,15 +40,24 @@ struct LocalWork {
 struct LuArgs;
 int launch_band_lu(const LuArgs& a, int nodes, hipStream_t stream);
 int launch_backsolve(const double* ab, const double* y, const int* uext, double* x, int n, int kl, int ku, int ldab,
-                     int nodes, int nR, int include_boundary, hipStream_t stream);
+                     int nodes, int nR, int include_boundary, hipStream_t stream, const int* nn = nullptr,
+                     const int* order = nullptr);
 // 0 = automatic (register-window LU where an instantiation covers kl),
 // 1 = force the one-column-per-step reference-order kernel.
 extern int local_lu_variant;
 // marks (nullable, 6 events): begin/end of the assembly, of the band factor
 // and of the solve (+ scatter), recorded on `stream`.
+// Uneven pitch-angle extents (device arrays, one entry per (P,R) node):
+// k0 = first active K column, n = nV*(nK-k0) unknowns, order = nodes sorted
+// by decreasing n (dispatch order of the factor and solve launches).
+struct NodeSizes {
+    const int* k0;
+    const int* n;
+    const int* order;
+};
 // cached: the factors live in a caller's cache (LAPACK layout incl. L).
 int launch_local(const verb_grid& g, double* psd, const LocalWork& w, bool factor, bool cached, int* status,
-                 hipStream_t stream, hipEvent_t* marks = nullptr);
+                 hipStream_t stream, hipEvent_t* marks = nullptr, const NodeSizes* sizes = nullptr);
 
 // Single-system LAPACK band routines on device buffers (one system).
 int launch_dgbtrf(int n, int kl, int ku, double* ab, int ldab, int64_t* ipiv, int* status, hipStream_t stream);

@@ -26,6 +26,9 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int GST = 5;
+// Largest sub-step count a launch accepts (outer Courant-0.25 steps of
+// Convection_2D, inner Courant-1 steps of QUICKEST6).
+constexpr double kMaxSubsteps = 1 << 20;
 
 struct LineBC {
     int lt, ut;        // boundary types
@@ -405,7 +408,13 @@ __global__ __launch_bounds__(kThreads) void convection_kernel(const verb_grid g,
         }
         double ns = (nsP < nsR) ? nsR : nsP;  // std::max
         if (single_outer) ns = 1;  // a bare QUICKEST6 call: inner sub-steps only
-        nsteps[tid] = (q < NQ) ? (int)ns : 0;
+        // A count beyond kMaxSubsteps (or an infinite one) could never finish:
+        // report it and leave the plane unchanged; NaN counts run no step, as
+        // the reference's loop `it < num_steps` does for NaN.
+        int nst = 0;
+        if (q < NQ && ns <= kMaxSubsteps) nst = (int)ns;
+        else if (q < NQ && ns > kMaxSubsteps) raise_error(status, VERB_ERR_COURANT);
+        nsteps[tid] = nst;
         dtq[tid] = dt_total / ns;
     }
     __syncthreads();
@@ -430,9 +439,13 @@ __global__ __launch_bounds__(kThreads) void convection_kernel(const verb_grid g,
                 dx = g.Rg[((size_t)a * g.nR + 1) * NQ + q] - g.Rg[((size_t)a * g.nR + 0) * NQ + q];
             }
             const double sd = (double)dtq[tq] * umax / dx / 1.0;
-            nin = (sd <= 1) ? 1 : (int)ceil(sd);
-            const double dti = dtq[tq] / nin;
-            if (fabs(umax / dx * dti) > 1) raise_error(status, VERB_ERR_COURANT);
+            if (sd <= 1) nin = 1;
+            else if (sd <= kMaxSubsteps) nin = (int)ceil(sd);
+            else if (sd > kMaxSubsteps) raise_error(status, VERB_ERR_COURANT);  // incl. infinite velocity
+            if (nin > 0) {
+                const double dti = dtq[tq] / nin;
+                if (fabs(umax / dx * dti) > 1) raise_error(status, VERB_ERR_COURANT);
+            }
         }
         (isP ? ninP : ninR)[ll] = nin;
     }

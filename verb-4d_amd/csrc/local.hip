@@ -27,9 +27,10 @@ constexpr int kAsmThreads = 256;
 
 // Node-level view of the grid slices that Diffusion_2D receives.
 struct NodeArgs {
-    int nx, ny;          // V, K
+    int nx, ny;          // V, active K extent of this node
+    int ld;              // row stride of the [V][K] node arrays (full K)
     double dt;           // sub-step dt
-    const double *x, *y, *Dxx, *Dyy, *Dxy, *G, *S, *L;  // [nx][ny], base of this node
+    const double *x, *y, *Dxx, *Dyy, *Dxy, *G, *S, *L;  // [nx][ld], first active K column
     const double *xl, *xu, *yl, *yu;                     // ny, ny, nx, nx
     int xlt, xut, ylt, yut;
     double src_mult;
@@ -37,8 +38,8 @@ struct NodeArgs {
 
 __device__ inline void sda2(double A[9], const NodeArgs& a, int ix, int iy, int dx1, int dy1, int dx2, int dy2,
                             const double* D, double mult) {
-    const int ny = a.ny;
-#define AT(arr, i, j) (arr)[(size_t)(i) * ny + (j)]
+    const int ld = a.ld;
+#define AT(arr, i, j) (arr)[(size_t)(i) * ld + (j)]
     const double dh1 = (AT(a.x, ix, iy) + AT(a.y, ix, iy)) - (AT(a.x, ix + dx1, iy + dy1) + AT(a.y, ix + dx1, iy + dy1));
     const double dh21 = (AT(a.x, ix, iy) + AT(a.y, ix, iy)) - (AT(a.x, ix + dx2, iy + dy2) + AT(a.y, ix + dx2, iy + dy2));
     const double dh22 = (AT(a.x, ix + dx1, iy + dy1) + AT(a.y, ix + dx1, iy + dy1)) -
@@ -59,25 +60,25 @@ __device__ inline bool assemble_row(const NodeArgs& a, int ix, int iy, double A[
     for (int s = 0; s < 9; s++) A[s] = 0.0;
     B0 = 0.0;
     C0 = 0.0;
-    const int nx = a.nx, ny = a.ny;
+    const int nx = a.nx, ny = a.ny, ld = a.ld;
     if (ix == 0 || ix == nx - 1 || iy == 0 || iy == ny - 1) {
         int type, s1;
         double dh;
         if (ix == 0) {
-            C0 = a.xl[iy]; s1 = 5; dh = a.x[(size_t)(ix + 1) * ny + iy] - a.x[(size_t)ix * ny + iy]; type = a.xlt;
+            C0 = a.xl[iy]; s1 = 5; dh = a.x[(size_t)(ix + 1) * ld + iy] - a.x[(size_t)ix * ld + iy]; type = a.xlt;
         } else if (ix == nx - 1) {
-            C0 = a.xu[iy]; s1 = 3; dh = a.x[(size_t)ix * ny + iy] - a.x[(size_t)(ix - 1) * ny + iy]; type = a.xut;
+            C0 = a.xu[iy]; s1 = 3; dh = a.x[(size_t)ix * ld + iy] - a.x[(size_t)(ix - 1) * ld + iy]; type = a.xut;
         } else if (iy == 0) {
-            C0 = a.yl[ix]; s1 = 7; dh = a.y[(size_t)ix * ny + iy + 1] - a.y[(size_t)ix * ny + iy]; type = a.ylt;
+            C0 = a.yl[ix]; s1 = 7; dh = a.y[(size_t)ix * ld + iy + 1] - a.y[(size_t)ix * ld + iy]; type = a.ylt;
         } else {
-            C0 = a.yu[ix]; s1 = 1; dh = a.y[(size_t)ix * ny + iy] - a.y[(size_t)ix * ny + iy - 1]; type = a.yut;
+            C0 = a.yu[ix]; s1 = 1; dh = a.y[(size_t)ix * ld + iy] - a.y[(size_t)ix * ld + iy - 1]; type = a.yut;
         }
         if (type == VERB_BC_CONSTANT_VALUE) { A[4] = 1; A[s1] = 0; }
         else if (type == VERB_BC_CONSTANT_DERIVATIVE) { A[4] = 1 / dh; A[s1] = -1 / dh; }
         else return false;
         return true;
     }
-    const size_t p = (size_t)ix * ny + iy;
+    const size_t p = (size_t)ix * ld + iy;
     A[4] += 1.0 / a.dt;
     B0 += 1.0 / a.dt;
     C0 += a.S[p] * a.src_mult;
@@ -97,14 +98,16 @@ __device__ inline bool assemble_row(const NodeArgs& a, int ix, int iy, double A[
     return true;
 }
 
-__device__ inline NodeArgs node_args(const verb_grid& g, int iP, int iR, double dt_sub) {
+// k0: first active pitch-angle column of the node; the node's system is the
+// reference's Diffusion_2D on the slice [V][k0:K] (y_size = K - k0).
+__device__ inline NodeArgs node_args(const verb_grid& g, int iP, int iR, double dt_sub, int k0) {
     NodeArgs a;
-    a.nx = g.nV; a.ny = g.nK; a.dt = dt_sub;
-    const size_t o = ((size_t)iP * g.nR + iR) * g.nV * g.nK;
+    a.nx = g.nV; a.ny = g.nK - k0; a.ld = g.nK; a.dt = dt_sub;
+    const size_t o = ((size_t)iP * g.nR + iR) * g.nV * g.nK + k0;
     a.x = g.Vg + o; a.y = g.Kg + o; a.Dxx = g.DVV + o; a.Dyy = g.DKK + o; a.Dxy = g.DVK + o;
     a.G = g.G_local + o; a.S = g.sources + o; a.L = g.losses_local + o;
-    a.xl = g.Vl + ((size_t)iP * g.nR + iR) * g.nK;
-    a.xu = g.Vu + ((size_t)iP * g.nR + iR) * g.nK;
+    a.xl = g.Vl + ((size_t)iP * g.nR + iR) * g.nK + k0;
+    a.xu = g.Vu + ((size_t)iP * g.nR + iR) * g.nK + k0;
     a.yl = g.Kl + ((size_t)iP * g.nR + iR) * g.nV;
     a.yu = g.Ku + ((size_t)iP * g.nR + iR) * g.nV;
     a.xlt = g.Vl_type; a.xut = g.Vu_type; a.ylt = g.Kl_type; a.yut = g.Ku_type;
@@ -123,37 +126,41 @@ __global__ __launch_bounds__(kAsmThreads) void local_assemble(const verb_grid g,
                                                              double* __restrict__ ab, double* __restrict__ b0,
                                                              double* __restrict__ c0, double* __restrict__ rhs,
                                                              double* __restrict__ diag, double dt_sub,
-                                                             int write_band, int premul, int* status) {
+                                                             int write_band, int premul, int* status,
+                                                             const int* __restrict__ k0s) {
     const int node = blockIdx.y;
     const int iP = node / g.nR, iR = node % g.nR;
     if (node_skipped(g, iR)) return;
-    const int nx = g.nV, ny = g.nK, n = nx * ny;
+    const int k0 = k0s ? k0s[node] : 0;
+    const int nx = g.nV, ny = g.nK - k0, n = nx * ny;
+    const int nfull = g.nV * g.nK;  // per-node stride of every workspace buffer
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int ix = p / ny, iy = p % ny;
     const int in = iy * nx + ix;
-    const NodeArgs a = node_args(g, iP, iR, dt_sub);
+    const NodeArgs a = node_args(g, iP, iR, dt_sub, k0);
     double A[9], B0, C0;
     if (!assemble_row(a, ix, iy, A, B0, C0)) raise_error(status, VERB_ERR_BC);
-    const size_t nn = (size_t)node * n;
+    const size_t nn = (size_t)node * nfull;
+    const size_t pp = nn + (size_t)ix * g.nK + k0 + iy;  // this point in the [V][K] node block
     b0[nn + in] = B0;
     c0[nn + in] = C0;
     if (premul) {
         // first sub-step's right-hand side: rhs = rhs*B0 + C0 (MatrixSolver.cpp:318-319)
-        double v = psd[nn + p] * B0;
+        double v = psd[pp] * B0;
         v += C0;
         rhs[nn + in] = v;
     } else {
-        rhs[nn + in] = psd[nn + p];
+        rhs[nn + in] = psd[pp];
     }
     if (diag) {
-        double* dn = diag + (size_t)node * 9 * n;
+        double* dn = diag + (size_t)node * 9 * nfull;
 #pragma unroll
         for (int sl = 0; sl < 9; sl++) dn[(size_t)sl * n + in] = A[sl];
     }
     if (write_band) {
         const int kl = nx + 1, ku = nx + 1, ldab = 2 * kl + ku + 1;
-        double* abn = ab + (size_t)node * ldab * n;
+        double* abn = ab + (size_t)node * ldab * nfull;
 #pragma unroll
         for (int dy = -1; dy <= 1; dy++)
 #pragma unroll
@@ -280,14 +287,17 @@ constexpr int kSolveThreads = 64;
 __global__ __launch_bounds__(kSolveThreads) void band_solve_kernel(
     const double* __restrict__ ab_all, const int64_t* __restrict__ ipiv_all, size_t ab_stride, int n, int kl,
     int ku, int ldab, const double* __restrict__ b0_all, const double* __restrict__ c0_all,
-    double* __restrict__ rhs_all, int nsub, int chunk, int nR, int include_boundary) {
+    double* __restrict__ rhs_all, int nsub, int chunk, int nR, int include_boundary,
+    const int* __restrict__ nn) {
     const int node = blockIdx.x;
     if (nR > 0 && !include_boundary && (node % nR == 0 || node % nR == nR - 1)) return;
+    const int stride = n;  // per-node buffer stride (the full node size)
+    if (nn) n = nn[node];
     const double* ab = ab_all + (size_t)node * ab_stride;
-    const int64_t* ipiv = ipiv_all + (size_t)node * n;
-    double* rhs = rhs_all + (size_t)node * n;
-    const double* B0 = b0_all ? b0_all + (size_t)node * n : nullptr;
-    const double* C0 = c0_all ? c0_all + (size_t)node * n : nullptr;
+    const int64_t* ipiv = ipiv_all + (size_t)node * stride;
+    double* rhs = rhs_all + (size_t)node * stride;
+    const double* B0 = b0_all ? b0_all + (size_t)node * stride : nullptr;
+    const double* C0 = c0_all ? c0_all + (size_t)node * stride : nullptr;
     const int tid = threadIdx.x;
     extern __shared__ double sh[];
     double* x = sh;           // n
@@ -362,15 +372,17 @@ __global__ __launch_bounds__(kSolveThreads) void band_solve_kernel(
 namespace {
 
 // rhs (energy-fast) -> psd slice (K-fast) for every node
-__global__ void scatter_psd(const verb_grid g, const double* __restrict__ rhs, double* __restrict__ psd) {
+__global__ void scatter_psd(const verb_grid g, const double* __restrict__ rhs, double* __restrict__ psd,
+                            const int* __restrict__ k0s) {
     const int node = blockIdx.y;
     const int iR = node % g.nR;
     if (node_skipped(g, iR)) return;
-    const int nx = g.nV, ny = g.nK, n = nx * ny;
+    const int k0 = k0s ? k0s[node] : 0;
+    const int nx = g.nV, ny = g.nK - k0, n = nx * ny, nfull = g.nV * g.nK;
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int ix = p / ny, iy = p % ny;
-    psd[(size_t)node * n + p] = rhs[(size_t)node * n + iy * nx + ix];
+    psd[(size_t)node * nfull + (size_t)ix * g.nK + k0 + iy] = rhs[(size_t)node * nfull + iy * nx + ix];
 }
 
 int solve_chunk(int n, int width) {
@@ -386,7 +398,10 @@ int solve_chunk(int n, int width) {
 }  // namespace
 
 int launch_local(const verb_grid& g, double* psd, const LocalWork& w, bool factor, bool cached, int* status,
-                 hipStream_t stream, hipEvent_t* marks) {
+                 hipStream_t stream, hipEvent_t* marks, const NodeSizes* sizes) {
+    const int* k0s = sizes ? sizes->k0 : nullptr;
+    const int* nns = sizes ? sizes->n : nullptr;
+    const int* order = sizes ? sizes->order : nullptr;
     if (g.nV < 3 || g.nK < 3) return VERB_ERR_ARG;
     const int nx = g.nV, ny = g.nK, n = nx * ny;
     const int kl = nx + 1, ku = nx + 1, ldab = 2 * kl + ku + 1;
@@ -402,7 +417,7 @@ int launch_local(const verb_grid& g, double* psd, const LocalWork& w, bool facto
         // plain path: factor with the forward elimination fused, then U x = y
         if (marks) hipEventRecord(marks[0], stream);
         hipLaunchKernelGGL(local_assemble, agrid, dim3(kAsmThreads), 0, stream, g, psd, w.ab, w.b0, w.c0, w.rhs,
-                           w.diag, dt_sub, 0, 1, status);
+                           w.diag, dt_sub, 0, 1, status, k0s);
         if (marks) {
             hipEventRecord(marks[1], stream);
             hipEventRecord(marks[2], stream);
@@ -411,15 +426,17 @@ int launch_local(const verb_grid& g, double* psd, const LocalWork& w, bool facto
         la.diag = w.diag; la.rhs = w.rhs; la.y = w.y; la.ab = w.ab; la.ipiv = w.ipiv; la.uext = w.uext;
         la.n = n; la.nx = nx; la.write_L = 0; la.status = status; la.nR = g.nR;
         la.include_boundary = g.include_boundary;
+        la.nn = nns; la.order = order;
         int rc = launch_band_lu(la, nodes, stream);
         if (rc) return rc;
         if (marks) {
             hipEventRecord(marks[3], stream);
             hipEventRecord(marks[4], stream);
         }
-        rc = launch_backsolve(w.ab, w.y, w.uext, w.rhs, n, kl, ku, ldab, nodes, g.nR, g.include_boundary, stream);
+        rc = launch_backsolve(w.ab, w.y, w.uext, w.rhs, n, kl, ku, ldab, nodes, g.nR, g.include_boundary, stream, nns,
+                              order);
         if (rc) return rc;
-        hipLaunchKernelGGL(scatter_psd, agrid, dim3(kAsmThreads), 0, stream, g, w.rhs, psd);
+        hipLaunchKernelGGL(scatter_psd, agrid, dim3(kAsmThreads), 0, stream, g, w.rhs, psd, k0s);
         if (marks) hipEventRecord(marks[5], stream);
         return hipGetLastError() == hipSuccess ? VERB_OK : VERB_ERR_HIP;
     }
@@ -428,9 +445,10 @@ int launch_local(const verb_grid& g, double* psd, const LocalWork& w, bool facto
         const size_t count = ab_stride * nodes;
         hipLaunchKernelGGL(fill_zero, dim3(4096), dim3(256), 0, stream, w.ab, count);
     }
+    if (sizes && !window) return VERB_ERR_ARG;  // uneven extents need the register-window LU
     hipLaunchKernelGGL(local_assemble, agrid, dim3(kAsmThreads), 0, stream, g, psd, w.ab, w.b0, w.c0, w.rhs,
                        (window && factor) ? w.diag : (double*)nullptr, dt_sub, (factor && !window) ? 1 : 0, 0,
-                       status);
+                       status, k0s);
     if (marks) {
         hipEventRecord(marks[1], stream);
         hipEventRecord(marks[2], stream);
@@ -441,6 +459,7 @@ int launch_local(const verb_grid& g, double* psd, const LocalWork& w, bool facto
         la.diag = w.diag; la.rhs = nullptr; la.y = nullptr; la.ab = w.ab; la.ipiv = w.ipiv; la.uext = w.uext;
         la.n = n; la.nx = nx; la.write_L = 1; la.status = status; la.nR = g.nR;
         la.include_boundary = g.include_boundary;
+        la.nn = nns; la.order = order;
         int rc = launch_band_lu(la, nodes, stream);
         if (rc) return rc;
     } else if (factor) {
@@ -455,8 +474,8 @@ int launch_local(const verb_grid& g, double* psd, const LocalWork& w, bool facto
     const size_t slds = sizeof(double) * ((size_t)n + (size_t)chunk * (kl + ku + 1));
     hipFuncSetAttribute((const void*)band_solve_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)slds);
     hipLaunchKernelGGL(band_solve_kernel, dim3(nodes), dim3(kSolveThreads), slds, stream, w.ab, w.ipiv, ab_stride,
-                       n, kl, ku, ldab, w.b0, w.c0, w.rhs, num_substeps, chunk, g.nR, g.include_boundary);
-    hipLaunchKernelGGL(scatter_psd, agrid, dim3(kAsmThreads), 0, stream, g, w.rhs, psd);
+                       n, kl, ku, ldab, w.b0, w.c0, w.rhs, num_substeps, chunk, g.nR, g.include_boundary, nns);
+    hipLaunchKernelGGL(scatter_psd, agrid, dim3(kAsmThreads), 0, stream, g, w.rhs, psd, k0s);
     if (marks) hipEventRecord(marks[5], stream);
     return hipGetLastError() == hipSuccess ? VERB_OK : VERB_ERR_HIP;
 }
@@ -477,7 +496,7 @@ int launch_dgbtrs(int n, int kl, int ku, const double* ab, int ldab, const int64
     const size_t slds = sizeof(double) * ((size_t)n + (size_t)chunk * (kl + ku + 1));
     hipFuncSetAttribute((const void*)band_solve_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)slds);
     hipLaunchKernelGGL(band_solve_kernel, dim3(1), dim3(kSolveThreads), slds, stream, ab, ipiv, (size_t)0, n, kl, ku,
-                       ldab, (const double*)nullptr, (const double*)nullptr, b, 1, chunk, 0, 1);
+                       ldab, (const double*)nullptr, (const double*)nullptr, b, 1, chunk, 0, 1, (const int*)nullptr);
     return hipGetLastError() == hipSuccess ? VERB_OK : VERB_ERR_HIP;
 }
 

@@ -106,6 +106,12 @@ _SIGNATURES = [
     ("verb_radial_diffusion_batched", ctypes.c_int, [_P, ctypes.POINTER(VerbGrid), ctypes.c_void_p]),
     ("verb_local_diffusion_batched", ctypes.c_int,
      [_P, ctypes.POINTER(VerbGrid), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
+    ("verb_local_diffusion_uneven", ctypes.c_int,
+     [_P, ctypes.POINTER(VerbGrid), ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.c_void_p, ctypes.c_void_p,
+      ctypes.c_int]),
+    ("verb_step_uneven", ctypes.c_int,
+     [_P, ctypes.POINTER(VerbGrid), ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int,
+      ctypes.c_int]),
     ("verb_lu_cache_doubles", ctypes.c_size_t, [ctypes.POINTER(VerbGrid)]),
     ("verb_lu_cache_ints", ctypes.c_size_t, [ctypes.POINTER(VerbGrid)]),
     ("verb_step", ctypes.c_int,

@@ -85,5 +85,20 @@ class DeviceGrid:
         self.ctx.check(lib().verb_local_diffusion_batched(self.ctx.handle, ctypes.byref(self.grid),
                                                           self.psd.data_ptr(), lu, piv, 1 if recompute_lu else 0))
 
+    def local_uneven(self, k0):
+        """Local diffusion with per-node first active K column k0 [P][R]."""
+        import numpy as np
+        k = np.ascontiguousarray(k0, dtype=np.int32).ravel()
+        self.ctx.check(lib().verb_local_diffusion_uneven(self.ctx.handle, ctypes.byref(self.grid), self.psd.data_ptr(),
+                                                         k.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), None, None,
+                                                         1))
+
+    def step_uneven(self, k0, run_convection=True, run_radial=True, run_local=True):
+        import numpy as np
+        k = np.ascontiguousarray(k0, dtype=np.int32).ravel()
+        self.ctx.check(lib().verb_step_uneven(self.ctx.handle, ctypes.byref(self.grid), self.psd.data_ptr(),
+                                              k.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), int(run_convection),
+                                              int(run_radial), int(run_local)))
+
     def check_status(self):
         return self.ctx.check_status()
