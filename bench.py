@@ -100,6 +100,18 @@ def cpu_baseline(w, psd, threads, target_s):
             "seconds_per_step": est}
 
 
+def copy_calibration():
+    import torch
+    n = 64 << 20
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print("copy calibration: %d bytes read and %d bytes written per copy launch" % (n * 8, n * 8), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,7 +121,13 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=3.0, help="target seconds per operator sample")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cached", action="store_true", help="also time steps that reuse cached LU factors")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 512 MiB device-to-device copy three times (a known-size kernel against "
+                         "which the FETCH_SIZE / WRITE_SIZE counters are scaled) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported (the multi-GPU "
               "sharding is a design note, not an implemented path)" % args.gpus, file=sys.stderr)

@@ -38,6 +38,7 @@
 namespace verb {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
+typedef double v16d __attribute__((ext_vector_type(16)));
 
 int local_lu_variant = 0;
 
@@ -45,13 +46,17 @@ namespace {
 
 constexpr int NB = 16;
 
+// Entry (r, c) of the assembled operator, 0 outside the 9 diagonals. The
+// load is issued unconditionally (from a clamped address) and selected
+// afterwards, so the loads of a tile reload are all in flight together.
 __device__ inline double a_orig(const double* dg, int n, int nx, int r, int c) {
-    if (r >= n || c >= n) return 0.0;
     const int off = c - r;
     const int dy = (off >= nx - 1) ? 1 : ((off <= 1 - nx) ? -1 : 0);
     const int dx = off - dy * nx;
-    if (dx < -1 || dx > 1) return 0.0;
-    return dg[(size_t)((dy + 1) * 3 + dx + 1) * n + r];
+    const bool ok = (r < n) & (c < n) & (dx >= -1) & (dx <= 1);
+    const size_t idx = ok ? (size_t)((dy + 1) * 3 + dx + 1) * n + r : 0;
+    const double v = dg[idx];
+    return ok ? v : 0.0;
 }
 
 // Accumulator layout of v_mfma_f64_16x16x4f64: lane j + 16*g, element r
@@ -59,10 +64,10 @@ __device__ inline double a_orig(const double* dg, int n, int nx, int r, int c) {
 // four 16-lane groups).
 __device__ inline int crow(int g, int r) { return 4 * r + g; }
 
-__device__ inline void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+__device__ inline double readlane_d(double v, int lane) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
 }
 
 // DPP move of a double within 16-lane rows (ctrl: a row-local DPP pattern).
@@ -73,32 +78,44 @@ __device__ inline double dpp_row(double v) {
     return __hiloint2double(hi, lo);
 }
 
-// Wavefront maximum: quad_perm [1,0,3,2], quad_perm [2,3,0,1],
-// row_half_mirror and row_mirror leave every lane of a 16-lane row with the
-// row maximum; the four row maxima are then combined.
+// Wavefront maximum, returned as a wave-uniform value: quad_perm [1,0,3,2],
+// quad_perm [2,3,0,1], row_half_mirror and row_mirror leave every lane of a
+// 16-lane row with the row maximum; the four row maxima are read with
+// readlane and combined.
 __device__ inline double wave_max(double v) {
     v = fmax(v, dpp_row<0xB1>(v));
     v = fmax(v, dpp_row<0x4E>(v));
     v = fmax(v, dpp_row<0x141>(v));
     v = fmax(v, dpp_row<0x140>(v));
-    const double r0 = __shfl(v, 0), r1 = __shfl(v, 16), r2 = __shfl(v, 32), r3 = __shfl(v, 48);
+    const double r0 = readlane_d(v, 0), r1 = readlane_d(v, 16), r2 = readlane_d(v, 32), r3 = readlane_d(v, 48);
     return fmax(fmax(r0, r1), fmax(r2, r3));
+}
+
+__device__ inline int wave_min_int(int v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = min(v, __shfl_xor(v, off));
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+__device__ inline int lanes_below(unsigned long long mask) {
+    return __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0));
 }
 
 template <int RB, int CB>
 struct Lds {
     static constexpr int W = NB * (CB - 1);  // window columns right of the panel
     static constexpr int PR = NB * RB;       // window rows
-    double pnl[PR * NB];                     // panel (full-row interchanges)
-    double pnl2[PR * NB];                    // copy for the dgbtf2-order multipliers
+    double pnl[NB * PR];                     // panel, column-major (full-row interchanges)
+    double pnl2[NB * PR];                    // factorisation scratch, then the dgbtf2-order multipliers
     double u12[NB * W];
-    double sb[2 * NB * W];                   // rows touched by the panel's interchanges
+    double sb[2 * NB * W];                   // pre-interchange contents of the rows that move
+    double stash[CB * 4 * 64];               // wavefront 0's accumulators while it factors
     double rw[PR];                           // right-hand-side window (index mod PR)
+    int slot[PR];                            // sb row holding window row i's old contents, -1: none
+    int dslot[PR];                           // sb row whose contents window row i takes, -1: unchanged
     int jp[NB];
     int uext[NB];                            // U row extents of the panel rows
-    int srow[2 * NB];
-    int ssrc[2 * NB];
-    int ns, ju, info;
+    int ju, info;
 };
 
 }  // namespace
@@ -124,7 +141,10 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgs a) {
     int* uext = a.uext ? a.uext + (size_t)node * stride : nullptr;
 
     const int tid = threadIdx.x;
-    const int w = tid >> 6, l = tid & 63, g = l >> 4, jj = l & 15;
+    // wavefront index as a uniform value: per-wavefront branches below are
+    // scalar, so the register allocator sees them as exclusive
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int l = tid & 63, g = l >> 4, jj = l & 15;
 
     v4d acc[CB];
     // initial window: row block w, column blocks 0..CB-1 in slots 0..CB-1
@@ -151,123 +171,185 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgs a) {
         for (int s = 0; s < CB; s++)
             if (s == s0) {
 #pragma unroll
-                for (int r = 0; r < 4; r++) S.pnl[(NB * lb + crow(g, r)) * NB + jj] = acc[s][r];
+                for (int r = 0; r < 4; r++) S.pnl[jj * PR + NB * lb + crow(g, r)] = acc[s][r];
             }
         __syncthreads();
 
-        // 2. panel factorisation (wavefront 0)
+        // 2. panel factorisation by wavefront 0, in registers. Lane l keeps
+        //    window rows l and l+64 as 16-wide vectors; an interchange only
+        //    swaps the rows' logical positions, the pivot row is broadcast
+        //    with readlane, the rank-1 update runs over the full row with the
+        //    pivot-row entries left of the step zeroed, and at the end every
+        //    row is stored at its final position (the result of full-row
+        //    interchanges).
         if (w == 0) {
+#pragma unroll
+            for (int s = 0; s < CB; s++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) S.stash[(s * 4 + r) * 64 + l] = acc[s][r];
             int ju = S.ju;
             int info = S.info;
+            constexpr int kNone = 1 << 20;
+            const int q0 = l, q1 = l + 64;
+            const bool h0 = q0 < PR, h1 = q1 < PR;
+            v16d x0, x1;
+#pragma unroll
+            for (int k = 0; k < NB; k++) {
+                x0[k] = h0 ? S.pnl[k * PR + q0] : 0.0;
+                x1[k] = h1 ? S.pnl[k * PR + q1] : 0.0;
+            }
+            int pos0 = h0 ? q0 : kNone, pos1 = h1 ? q1 : kNone;
+            double rh0 = h0 ? S.rw[(j0 + q0) % PR] : 0.0;
+            double rh1 = h1 ? S.rw[(j0 + q1) % PR] : 0.0;
             for (int c = 0; c < nc; c++) {
                 const int j = j0 + c;
                 const int km = min(kl, n - 1 - j);
-                const double v1 = (l <= km) ? fabs(S.pnl[(c + l) * NB + c]) : -1.0;
-                const double v2 = (l + 64 <= km) ? fabs(S.pnl[(c + l + 64) * NB + c]) : -1.0;
-                const double m = wave_max(fmax(v1, v2));
-                const unsigned long long b1 = __ballot(v1 == m && l <= km);
-                const unsigned long long b2 = __ballot(v2 == m && l + 64 <= km);
-                int jp = 0;
-                if (b1) jp = __ffsll(b1) - 1;
-                else if (b2) jp = 64 + __ffsll(b2) - 1;
+                const double xc0 = x0[c], xc1 = x1[c];
+                // candidates: logical rows c..c+km
+                const bool c0 = (pos0 >= c) & (pos0 <= c + km);
+                const bool c1 = (pos1 >= c) & (pos1 <= c + km);
+                const double v0 = c0 ? fabs(xc0) : -1.0;
+                const double v1 = c1 ? fabs(xc1) : -1.0;
+                const double m = wave_max(fmax(v0, v1));
+                const bool t0 = c0 & (v0 == m), t1 = c1 & (v1 == m);
+                const unsigned long long tb0 = __ballot(t0), tb1 = __ballot(t1);
+                int jpos;
+                if (__popcll(tb0) + __popcll(tb1) == 1) {
+                    // a unique largest entry (the common case)
+                    jpos = tb0 ? __builtin_amdgcn_readlane(pos0, __ffsll(tb0) - 1)
+                               : __builtin_amdgcn_readlane(pos1, __ffsll(tb1) - 1);
+                } else if (tb0 | tb1) {
+                    // ties: LAPACK takes the first logical row
+                    jpos = wave_min_int(min(t0 ? pos0 : kNone, t1 ? pos1 : kNone));
+                } else {
+                    jpos = c;
+                }
+                const unsigned long long pb0 = __ballot(pos0 == jpos);
+                const unsigned long long pb1 = __ballot(pos1 == jpos);
+                const int jp = jpos - c;
                 if (l == 0) ipiv[j] = (int64_t)(j + jp + 1);
-                const double piv = S.pnl[(c + jp) * NB + c];
+                const bool in1 = (pb0 == 0);
+                const int pl = in1 ? (__ffsll(pb1) - 1) : (__ffsll(pb0) - 1);
+                const double piv = readlane_d(in1 ? xc1 : xc0, pl);
                 if (piv != 0.0) {
                     ju = max(ju, min(j + ku + jp, n - 1));
-                    if (jp != 0) {
-                        if (l < NB) {
-                            const double t = S.pnl[c * NB + l];
-                            S.pnl[c * NB + l] = S.pnl[(c + jp) * NB + l];
-                            S.pnl[(c + jp) * NB + l] = t;
-                        }
-                        if (l == 0) {
-                            const double t = S.rw[j % PR];
-                            S.rw[j % PR] = S.rw[(j + jp) % PR];
-                            S.rw[(j + jp) % PR] = t;
+                    // pivot row right of the step (zero at and left of c)
+                    double pr[NB];
+                    double prh;
+                    if (in1) {
+#pragma unroll
+                        for (int k = 0; k < NB; k++) pr[k] = readlane_d(x1[k], pl);
+                        prh = readlane_d(rh1, pl);
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < NB; k++) pr[k] = readlane_d(x0[k], pl);
+                        prh = readlane_d(rh0, pl);
+                    }
+#pragma unroll
+                    for (int k = 0; k < NB; k++) pr[k] = (k > c) ? pr[k] : 0.0;
+                    // interchange: pivot row -> position c, the row at c -> jpos
+                    pos0 = (pos0 == jpos) ? c : ((pos0 == c) ? jpos : pos0);
+                    pos1 = (pos1 == jpos) ? c : ((pos1 == c) ? jpos : pos1);
+                    const double rinv = 1.0 / piv;
+                    const bool e0 = (pos0 > c) & (pos0 <= c + km);
+                    const bool e1 = (pos1 > c) & (pos1 <= c + km);
+                    // multipliers; 0 for rows the step does not touch
+                    const double L0 = e0 ? xc0 * rinv : 0.0;
+                    const double L1 = e1 ? xc1 * rinv : 0.0;
+                    x0[c] = e0 ? L0 : xc0;
+                    x1[c] = e1 ? L1 : xc1;
+                    const double tb = -prh;
+                    if (tb != 0.0) {
+                        rh0 = e0 ? fma(L0, tb, rh0) : rh0;
+                        rh1 = e1 ? fma(L1, tb, rh1) : rh1;
+                    }
+                    if (fabs(rinv) <= 1.7976931348623157e308) {
+                        // finite multipliers: fma(0, t, x) == x for untouched
+                        // rows and fma(L, 0, x) == x for a zero pivot-row entry
+#pragma unroll
+                        for (int k = 0; k < NB; k++) {
+                            x0[k] = fma(L0, -pr[k], x0[k]);
+                            x1[k] = fma(L1, -pr[k], x1[k]);
                         }
                     } else {
-                        jp = 0;
-                    }
-                    wave_sync();
-                    const double rinv = 1.0 / S.pnl[c * NB + c];
-                    double L0 = 0.0, L1 = 0.0;
-                    const bool h0 = (l + 1 <= km), h1 = (l + 65 <= km);
-                    if (h0) {
-                        L0 = S.pnl[(c + 1 + l) * NB + c] * rinv;
-                        S.pnl[(c + 1 + l) * NB + c] = L0;
-                    }
-                    if (h1) {
-                        L1 = S.pnl[(c + 65 + l) * NB + c] * rinv;
-                        S.pnl[(c + 65 + l) * NB + c] = L1;
-                    }
-                    const double tb = -S.rw[j % PR];
-                    if (tb != 0.0) {
-                        if (h0) S.rw[(j + 1 + l) % PR] += L0 * tb;
-                        if (h1) S.rw[(j + 65 + l) % PR] += L1 * tb;
-                    }
-                    for (int cc = c + 1; cc < NB; cc++) {
-                        const double temp = -S.pnl[c * NB + cc];
-                        if (temp != 0.0) {
-                            if (h0) S.pnl[(c + 1 + l) * NB + cc] += L0 * temp;
-                            if (h1) S.pnl[(c + 65 + l) * NB + cc] += L1 * temp;
+#pragma unroll
+                        for (int k = 0; k < NB; k++) {
+                            const double temp = -pr[k];
+                            if (temp != 0.0) {
+                                x0[k] = e0 ? fma(L0, temp, x0[k]) : x0[k];
+                                x1[k] = e1 ? fma(L1, temp, x1[k]) : x1[k];
+                            }
                         }
                     }
+                    if (l == 0) S.jp[c] = jp;
                 } else {
-                    jp = 0;
                     if (info == 0) info = j + 1;
+                    if (l == 0) S.jp[c] = 0;
                 }
                 if (l == 0) {
-                    S.jp[c] = jp;
                     S.uext[c] = ju;
                     if (uext) uext[j] = ju;
                 }
-                wave_sync();
             }
-            for (int c = nc + l; c < NB; c += 64) S.jp[c] = 0;
+            // rows to their final positions
+            if (h0) {
+#pragma unroll
+                for (int k = 0; k < NB; k++) S.pnl[k * PR + pos0] = x0[k];
+                S.rw[(j0 + pos0) % PR] = rh0;
+            }
+            if (h1) {
+#pragma unroll
+                for (int k = 0; k < NB; k++) S.pnl[k * PR + pos1] = x1[k];
+                S.rw[(j0 + pos1) % PR] = rh1;
+            }
+            // sb rows: panel rows keep their index; rows below the panel that
+            // moved (pivot rows pulled up, at most one per column) are numbered
+            // after them. dslot gives each window row the sb row it takes.
+            const bool mv0 = h0 && q0 >= NB && pos0 != q0;
+            const bool mv1 = h1 && pos1 != q1;
+            const unsigned long long mb0 = __ballot(mv0), mb1 = __ballot(mv1);
+            const int sl0 = (q0 < NB) ? q0 : (mv0 ? NB + lanes_below(mb0) : -1);
+            const int sl1 = mv1 ? NB + __popcll(mb0) + lanes_below(mb1) : -1;
+            if (h0) {
+                S.slot[q0] = sl0;
+                S.dslot[pos0] = (pos0 < NB || pos0 != q0) ? sl0 : -1;
+            }
+            if (h1) {
+                S.slot[q1] = sl1;
+                S.dslot[pos1] = (pos1 < NB || pos1 != q1) ? sl1 : -1;
+            }
+            if (l < NB && l >= nc) S.jp[l] = 0;
             if (l == 0) {
                 S.ju = ju;
                 S.info = info;
-                // net permutation of the rows the interchanges touched
-                int rows[2 * NB], cont[2 * NB];
-                int cnt = NB;
-                for (int c = 0; c < NB; c++) { rows[c] = c; cont[c] = c; }
-                for (int c = 0; c < nc; c++) {
-                    const int t = c + S.jp[c];
-                    int k = -1;
-                    for (int q = 0; q < cnt; q++)
-                        if (rows[q] == t) k = q;
-                    if (k < 0) { rows[cnt] = t; cont[cnt] = t; k = cnt++; }
-                    const int tmp = cont[c]; cont[c] = cont[k]; cont[k] = tmp;
-                }
-                for (int q = 0; q < cnt; q++) {
-                    int src = 0;
-                    for (int m = 0; m < cnt; m++)
-                        if (rows[m] == cont[q]) src = m;
-                    S.srow[q] = rows[q];
-                    S.ssrc[q] = src;
-                }
-                S.ns = cnt;
             }
+#pragma unroll
+            for (int s = 0; s < CB; s++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) acc[s][r] = S.stash[(s * 4 + r) * 64 + l];
         }
         __syncthreads();
         const int ju = S.ju;
-        const int ns = S.ns;
 
-        // 3a. rows touched by the interchanges -> LDS (columns right of the panel)
-        for (int q = 0; q < ns; q++) {
-            const int row = S.srow[q];
-            if (row / NB == lb && (row & 3) == g) {
-                const int r = (row & 15) >> 2;
+        // 3a. pre-interchange contents of the rows that move -> sb (columns
+        //     right of the panel)
+        {
+            int sl[4];
 #pragma unroll
-                for (int s = 0; s < CB; s++) {
-                    if (s == s0) continue;
-                    const int lc = (s - s0 + CB) % CB;
-                    double v = (r == 0) ? acc[s][0] : (r == 1) ? acc[s][1] : (r == 2) ? acc[s][2] : acc[s][3];
-                    S.sb[q * W + NB * (lc - 1) + jj] = v;
+            for (int r = 0; r < 4; r++) sl[r] = S.slot[NB * lb + crow(g, r)];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                if (sl[r] >= 0) {
+#pragma unroll
+                    for (int s = 0; s < CB; s++) {
+                        if (s == s0) continue;
+                        const int lc = (s - s0 + CB) % CB;
+                        S.sb[sl[r] * W + NB * (lc - 1) + jj] = acc[s][r];
+                    }
                 }
             }
         }
-        if (w == 0) {
+        if (a.write_L && w == 0) {
             for (int i = l; i < PR * NB; i += 64) S.pnl2[i] = S.pnl[i];
         }
         __syncthreads();
@@ -276,33 +358,31 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgs a) {
         for (int t = tid; t < W; t += RB * 64) {
             double u[NB];
 #pragma unroll
-            for (int c = 0; c < NB; c++) u[c] = (c < nc) ? S.sb[S.ssrc[c] * W + t] : 0.0;
+            for (int c = 0; c < NB; c++) u[c] = (c < nc) ? S.sb[S.dslot[c] * W + t] : 0.0;
 #pragma unroll
             for (int c = 1; c < NB; c++)
 #pragma unroll
                 for (int m = 0; m < c; m++) {
                     const double temp = -u[m];
-                    if (temp != 0.0) u[c] += S.pnl[c * NB + m] * temp;
+                    if (temp != 0.0) u[c] += S.pnl[m * PR + c] * temp;
                 }
 #pragma unroll
             for (int c = 0; c < NB; c++) S.u12[c * W + t] = u[c];
         }
-        // 3c. displaced rows below the panel take their new contents
-        for (int q = NB; q < ns; q++) {
-            const int row = S.srow[q];
-            if (S.ssrc[q] == q) continue;
-            if (row / NB == lb && (row & 3) == g) {
-                const int r = (row & 15) >> 2;
-                const int src = S.ssrc[q];
+        // 3c. rows below the panel whose contents changed take them from sb
+        if (lb != 0) {
+            int ds[4];
 #pragma unroll
-                for (int s = 0; s < CB; s++) {
-                    if (s == s0) continue;
-                    const int lc = (s - s0 + CB) % CB;
-                    const double v = S.sb[src * W + NB * (lc - 1) + jj];
-                    if (r == 0) acc[s][0] = v;
-                    else if (r == 1) acc[s][1] = v;
-                    else if (r == 2) acc[s][2] = v;
-                    else acc[s][3] = v;
+            for (int r = 0; r < 4; r++) ds[r] = S.dslot[NB * lb + crow(g, r)];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                if (ds[r] >= 0) {
+#pragma unroll
+                    for (int s = 0; s < CB; s++) {
+                        if (s == s0) continue;
+                        const int lc = (s - s0 + CB) % CB;
+                        acc[s][r] = S.sb[ds[r] * W + NB * (lc - 1) + jj];
+                    }
                 }
             }
         }
@@ -313,7 +393,7 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgs a) {
             const int rowbase = NB * lb;
             double an[4];
 #pragma unroll
-            for (int kk = 0; kk < 4; kk++) an[kk] = -S.pnl[(rowbase + jj) * NB + 4 * kk + g];
+            for (int kk = 0; kk < 4; kk++) an[kk] = -S.pnl[(4 * kk + g) * PR + rowbase + jj];
 #pragma unroll
             for (int s = 0; s < CB; s++) {
                 if (s == s0) continue;
@@ -337,17 +417,22 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgs a) {
                         acc[s][r] = a_orig(dg, n, nx, NB * rb + crow(g, r), NB * (p + CB) + jj);
                 }
         } else {
-            // U rows j0..j0+nc-1: columns r..ext(r); beyond ext(r) the row is
-            // zero (a zero-filled band stays exact)
-            for (int e = l; e < NB * (NB + W); e += 64) {
-                const int c = e % NB;
-                const int col_off = e / NB;  // 0..NB+W-1, relative to j0
-                if (c >= nc) continue;
+            // U rows j0..j0+nc-1: lane l writes row c = l%16 at columns
+            // j0 + l/16 + 4i up to the row's extent; beyond it the row is zero
+            // (a zero-filled band stays exact)
+            {
+                const int c = jj;
                 const int r = j0 + c;
-                const int col = j0 + col_off;
-                if (col < r || col > S.uext[c]) continue;
-                const double v = (col_off < NB) ? S.pnl[c * NB + col_off] : S.u12[c * W + (col_off - NB)];
-                ab[(size_t)col * ldab + kv + r - col] = v;
+                const int ue = (c < nc) ? S.uext[c] : -1;
+#pragma unroll
+                for (int i = 0; i < (NB + W) / 4; i++) {
+                    const int col_off = g + 4 * i;
+                    const int col = j0 + col_off;
+                    if (col >= r && col <= ue) {
+                        const double v = (i < NB / 4) ? S.pnl[col_off * PR + c] : S.u12[c * W + (col_off - NB)];
+                        ab[(size_t)col * ldab + kv + r - col] = v;
+                    }
+                }
             }
             if (yv)
                 for (int c = l; c < nc; c += 64) yv[j0 + c] = S.rw[(j0 + c) % PR];
@@ -357,16 +442,18 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgs a) {
                 for (int c = nc - 1; c >= 0; c--) {
                     const int jp = S.jp[c];
                     if (jp != 0 && l < c) {
-                        const double t = S.pnl2[c * NB + l];
-                        S.pnl2[c * NB + l] = S.pnl2[(c + jp) * NB + l];
-                        S.pnl2[(c + jp) * NB + l] = t;
+                        const double t = S.pnl2[l * PR + c];
+                        S.pnl2[l * PR + c] = S.pnl2[l * PR + c + jp];
+                        S.pnl2[l * PR + c + jp] = t;
                     }
-                    wave_sync();
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 }
                 for (int c = 0; c < nc; c++) {
                     const int km = min(kl, n - 1 - (j0 + c));
                     for (int i = l; i < kl; i += 64)
-                        ab[(size_t)(j0 + c) * ldab + kv + 1 + i] = (i < km) ? S.pnl2[(c + 1 + i) * NB + c] : 0.0;
+                        ab[(size_t)(j0 + c) * ldab + kv + 1 + i] = (i < km) ? S.pnl2[c * PR + c + 1 + i] : 0.0;
                 }
             }
             // the row block entering at the bottom, for panel p+1's window
