@@ -12,6 +12,9 @@ Prints one JSON line. Besides the driver's fields it carries
   roofline      the dominant kernel's achieved rate (algorithmic work per
                 launch / its average launch time, HIP events on the library
                 stream) against the MI355X peak
+                and its HBM traffic per launch, taken from the FETCH_SIZE /
+                WRITE_SIZE profile of the same command kept in
+                profiles/step_production_grid.md (null when absent)
   cpu_baseline  the reference's own CPU code (oracle/_ref, compiled from the
                 PRBEM/VERB-4D sources, OpenMP + OpenBLAS) timed on this
                 host on a bounded sample of the same step
@@ -31,6 +34,25 @@ sys.path.insert(0, os.path.join(ROOT, "verb-4d_amd"))
 METRIC = "full-grid steps/s at 29×24×101×91, ≤1e-8 rel err vs FP64; % of roofline"
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
+
+
+PHASE_KERNEL = {"local_factor": "band_lu_window", "local_solve": "band_backsolve"}
+
+
+def profiled_traffic(phase):
+    """HBM bytes per launch of the phase's kernel, as measured by the rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs summarised in profiles/step_production_grid.md
+    (calibrated against a copy of known size), or None."""
+    kern = PHASE_KERNEL.get(phase)
+    path = os.path.join(ROOT, "profiles", "step_production_grid.md")
+    if kern is None or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kern:
+                return float(parts[2])
+    return None
 
 
 def lu_flops(n, kl, ku):
@@ -57,8 +79,8 @@ def phase_work(w):
         # 8 inputs + 3 outputs per point of the node slices
         "local_assemble": (88.0 * pts, "B", "hbm"),
         "local_factor": (lu_flops(n, kl, ku) * nodes, "flop", "mfma"),
-        # factors read once: (2kl+ku+1) doubles per column, plus rhs r/w
-        "local_solve": ((2 * kl + ku + 1) * 8.0 * n * nodes + 16.0 * pts, "B", "hbm"),
+        # U read once: kl+ku+1 doubles per column, plus right-hand side and PSD r/w
+        "local_solve": ((kl + ku + 1) * 8.0 * n * nodes + 16.0 * pts, "B", "hbm"),
     }
 
 
@@ -195,7 +217,7 @@ def main():
     d = phase_out[dom]
     roofline = {"bound": bound, "achieved": d["achieved"],
                 "peak": PEAK_FP64_TFLOPS if unit == "flop" else PEAK_HBM_GBS, "unit": d["unit"],
-                "frac": d["frac"], "traffic": None, "kernel": dom}
+                "frac": d["frac"], "traffic": profiled_traffic(dom), "kernel": dom}
 
     cached = None
     if args.cached:
