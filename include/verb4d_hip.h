@@ -63,7 +63,7 @@ size_t verb_ctx_scratch_bytes(verb_ctx* ctx);
  * No reference counterpart: the reference only times the whole run
  * (src/VERB4D_Solver.cpp:429-431, 1311-1312). */
 enum { VERB_PHASE_CONVECTION = 0, VERB_PHASE_RADIAL = 1, VERB_PHASE_LOCAL_ASSEMBLE = 2, VERB_PHASE_LOCAL_FACTOR = 3,
-       VERB_PHASE_LOCAL_SOLVE = 4, VERB_NUM_PHASES = 5 };
+       VERB_PHASE_LOCAL_SOLVE = 4, VERB_PHASE_LOCAL_KRYLOV = 5, VERB_NUM_PHASES = 6 };
 int verb_ctx_set_timing(verb_ctx* ctx, int enable);
 int verb_ctx_get_timings(verb_ctx* ctx, double* ms, long* counts);
 
@@ -180,6 +180,38 @@ int verb_step(verb_ctx* ctx, const verb_grid* g, double* psd, int run_convection
 /* Pulls the device-side status word of the last batched launches (Courant
  * violation, zero pivot) and synchronises the stream. */
 int verb_check_status(verb_ctx* ctx);
+
+/* ------------------------------------------------------------------------
+ * Local diffusion by matrix-free GMRES. No counterpart in the reference
+ * (src/Diffusion_2D.cpp:144-148 always factors with dgbtrf_); it solves the
+ * same system A x = B0*psd + C0 per (P,R) node for a single sub-step
+ * (dt <= sub_dt). FP32 restarted GMRES cycles, right-preconditioned by the
+ * exact pitch-angle tridiagonal blocks, correct an FP64 iterate; a node is
+ * converged when the FP64 residual satisfies max|b - A x| <= tol * max|b|.
+ * Nodes that are not converged after max_cycles cycles are factored by the
+ * band LU (the verb_local_diffusion_batched path) instead. The initial
+ * iterate is the incoming PSD. stats (nullable) synchronises the stream.
+ * ------------------------------------------------------------------------ */
+typedef struct {
+    double tol;       /* FP64 residual tolerance, relative to max|b| */
+    int restart;      /* Krylov vectors per FP32 cycle, 1..24 */
+    int max_cycles;   /* cycles before a node falls back to the band LU */
+    double inner_tol; /* residual reduction that ends an FP32 cycle early */
+} verb_gmres_params;
+
+typedef struct {
+    int nodes;            /* systems solved (nodes skipped by include_boundary excluded) */
+    int converged;        /* met tol by GMRES */
+    int fallback;         /* solved by the band LU instead */
+    long long iterations; /* inner iterations, summed over the solved nodes */
+    int max_iterations;   /* inner iterations of the slowest node */
+} verb_gmres_stats;
+
+int verb_local_diffusion_gmres(verb_ctx* ctx, const verb_grid* g, double* psd, const verb_gmres_params* params,
+                               verb_gmres_stats* stats);
+/* convection + radial + verb_local_diffusion_gmres */
+int verb_step_gmres(verb_ctx* ctx, const verb_grid* g, double* psd, const verb_gmres_params* params,
+                    verb_gmres_stats* stats);
 
 #ifdef __cplusplus
 }

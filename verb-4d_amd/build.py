@@ -23,6 +23,7 @@ SOURCES = [
     ("convection.hip", ["-ffp-contract=off"]),
     ("local.hip", ["-ffp-contract=off"]),
     ("band_lu.hip", []),
+    ("gmres.hip", []),
     ("api.hip", []),
 ]
 

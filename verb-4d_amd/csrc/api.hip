@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "common.h"
+#include "gmres.h"
 
 namespace verb {
 int launch_convection_mode(const verb_grid& g, double* psd, int* status, hipStream_t stream, int single_outer);
@@ -344,6 +345,73 @@ int verb_step_uneven(verb_ctx* c, const verb_grid* g, double* psd, const int* k0
     if (!rc && run_radial && g->nR >= 3) rc = verb_radial_diffusion_batched(c, g, psd);
     if (!rc && run_local && g->nV >= 3 && g->nK >= 3)
         rc = verb_local_diffusion_uneven(c, g, psd, k0, nullptr, nullptr, 1);
+    return rc;
+}
+
+constexpr int kSlotGmres = 60;  // scratch slots 60..66 for the GMRES workspace
+
+int verb_local_diffusion_gmres(verb_ctx* c, const verb_grid* g, double* psd, const verb_gmres_params* prm,
+                               verb_gmres_stats* stats) {
+    if (!c || !grid_ok(g) || !psd || !prm) return VERB_ERR_ARG;
+    if (!(g->sub_dt > 0) || (int)(g->dt / g->sub_dt) > 1) return set_err(c, VERB_ERR_ARG, "GMRES path: one sub-step only");
+    if (prm->restart < 1 || prm->restart > verb::kGmMaxM || prm->max_cycles < 0 || !(prm->tol > 0))
+        return set_err(c, VERB_ERR_ARG, "GMRES parameters out of range");
+    if (g->nV + 1 > 112) return set_err(c, VERB_ERR_ARG, "GMRES path: fallback LU needs nV <= 111");
+    const size_t n = (size_t)g->nV * g->nK;
+    const int nodes = g->nP * g->nR;
+    std::lock_guard<std::mutex> lk(c->mu);
+    LocalBuffers lb = local_buffers(c, g, nullptr, nullptr, kSlotLocal);
+    if (lb.rc) return lb.rc;
+    verb::GmresWork gw;
+    gw.x = (double*)scratch(c, kSlotGmres + 0, n * nodes * sizeof(double));
+    gw.d32 = (float*)scratch(c, kSlotGmres + 1, 9 * n * nodes * sizeof(float));
+    gw.pf = (float*)scratch(c, kSlotGmres + 2, 3 * n * nodes * sizeof(float));
+    gw.V = (float*)scratch(c, kSlotGmres + 3, (size_t)(verb::kGmMaxM + 1) * n * nodes * sizeof(float));
+    gw.z = (float*)scratch(c, kSlotGmres + 4, 2 * n * nodes * sizeof(float));
+    gw.conv = (int*)scratch(c, kSlotGmres + 5, nodes * sizeof(int));
+    gw.iters = (int*)scratch(c, kSlotGmres + 6, nodes * sizeof(int));
+    if (!gw.x || !gw.d32 || !gw.pf || !gw.V || !gw.z || !gw.conv || !gw.iters)
+        return set_err(c, VERB_ERR_NOMEM, "device allocation failed");
+    hipEvent_t ev[6];
+    hipEvent_t* marks = nullptr;
+    if (c->timing) {
+        for (auto& e : ev) e = new_event();
+        marks = ev;
+    }
+    int rc = verb::launch_local_gmres(*g, psd, lb.w, gw, prm->tol, prm->restart, prm->max_cycles, prm->inner_tol,
+                                      c->status, c->stream, marks);
+    if (marks) {
+        c->marks.push_back({VERB_PHASE_LOCAL_ASSEMBLE, ev[0], ev[1]});
+        c->marks.push_back({VERB_PHASE_LOCAL_KRYLOV, ev[2], ev[3]});
+        c->marks.push_back({VERB_PHASE_LOCAL_FACTOR, ev[4], ev[5]});
+    }
+    if (rc) return set_err(c, rc, "GMRES local diffusion launch rejected");
+    if (stats) {
+        std::vector<int> conv(nodes), it(nodes);
+        HIPCHK(c, hipMemcpyAsync(conv.data(), gw.conv, nodes * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(it.data(), gw.iters, nodes * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        verb_gmres_stats st{0, 0, 0, 0, 0};
+        for (int i = 0; i < nodes; i++) {
+            const int iR = i % g->nR;
+            if (!g->include_boundary && (iR == 0 || iR == g->nR - 1)) continue;
+            st.nodes++;
+            if (conv[i]) st.converged++;
+            else st.fallback++;
+            st.iterations += it[i];
+            if (it[i] > st.max_iterations) st.max_iterations = it[i];
+        }
+        *stats = st;
+    }
+    return VERB_OK;
+}
+
+int verb_step_gmres(verb_ctx* c, const verb_grid* g, double* psd, const verb_gmres_params* prm,
+                    verb_gmres_stats* stats) {
+    int rc = VERB_OK;
+    if (3 < g->nP || 3 < g->nR) rc = verb_convection_batched(c, g, psd);
+    if (!rc && g->nR >= 3) rc = verb_radial_diffusion_batched(c, g, psd);
+    if (!rc && g->nV >= 3 && g->nK >= 3) rc = verb_local_diffusion_gmres(c, g, psd, prm, stats);
     return rc;
 }
 

@@ -16,6 +16,7 @@ struct LuArgs {
     int nR, include_boundary;  // node skip rule (nR = 0: factor every node)
     const int* nn;             // per-node unknowns (null: n for every node); buffers keep stride n
     const int* order;          // dispatch order of the nodes (null: identity)
+    const int* done = nullptr; // nodes with done[node] != 0 are left alone (null: none)
 };
 
 }  // namespace verb

@@ -178,6 +178,7 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgs a) {
 
     const int node = a.order ? a.order[blockIdx.x] : blockIdx.x;
     if (a.nR > 0 && !a.include_boundary && (node % a.nR == 0 || node % a.nR == a.nR - 1)) return;
+    if (a.done && a.done[node]) return;
     const int stride = a.n;
     const int n = a.nn ? a.nn[node] : a.n, nx = a.nx;
     const int kl = nx + 1, ku = nx + 1, kv = kl + ku, ldab = 2 * kl + ku + 1;
@@ -544,9 +545,11 @@ __global__ __launch_bounds__(kBsThreads) void band_backsolve(const double* __res
                                                              double* __restrict__ x_all, int n, int kl, int ku,
                                                              int ldab, int nR, int include_boundary,
                                                              const int* __restrict__ nn,
-                                                             const int* __restrict__ order) {
+                                                             const int* __restrict__ order,
+                                                             const int* __restrict__ done) {
     const int node = order ? order[blockIdx.x] : blockIdx.x;
     if (nR > 0 && !include_boundary && (node % nR == 0 || node % nR == nR - 1)) return;
+    if (done && done[node]) return;
     const int stride = n;
     if (nn) n = nn[node];
     const int kv = kl + ku;
@@ -641,7 +644,8 @@ __global__ __launch_bounds__(kBsThreads) void band_backsolve(const double* __res
 }
 
 int launch_backsolve(const double* ab, const double* y, const int* uext, double* x, int n, int kl, int ku, int ldab,
-                     int nodes, int nR, int include_boundary, hipStream_t stream, const int* nn, const int* order) {
+                     int nodes, int nR, int include_boundary, hipStream_t stream, const int* nn, const int* order,
+                     const int* done) {
     (void)uext;  // U rows are stored up to kl+ku past the diagonal, so extents are not needed
     if (kl + ku + NB > NB * kBsMaxI) return VERB_ERR_ARG;
     const int WS = kl + ku + NB;
@@ -652,7 +656,7 @@ int launch_backsolve(const double* ab, const double* y, const int* uext, double*
     if (lds + 256 > (size_t)max_lds) return VERB_ERR_ARG;
     hipFuncSetAttribute((const void*)band_backsolve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(band_backsolve, dim3(nodes), dim3(kBsThreads), lds, stream, ab, y, x, n, kl, ku, ldab, nR,
-                       include_boundary, nn, order);
+                       include_boundary, nn, order, done);
     return hipGetLastError() == hipSuccess ? VERB_OK : VERB_ERR_HIP;
 }
 

@@ -41,7 +41,7 @@ struct LuArgs;
 int launch_band_lu(const LuArgs& a, int nodes, hipStream_t stream);
 int launch_backsolve(const double* ab, const double* y, const int* uext, double* x, int n, int kl, int ku, int ldab,
                      int nodes, int nR, int include_boundary, hipStream_t stream, const int* nn = nullptr,
-                     const int* order = nullptr);
+                     const int* order = nullptr, const int* done = nullptr);
 // 0 = automatic (register-window LU where an instantiation covers kl),
 // 1 = force the one-column-per-step reference-order kernel.
 extern int local_lu_variant;
@@ -58,6 +58,22 @@ struct NodeSizes {
 // cached: the factors live in a caller's cache (LAPACK layout incl. L).
 int launch_local(const verb_grid& g, double* psd, const LocalWork& w, bool factor, bool cached, int* status,
                  hipStream_t stream, hipEvent_t* marks = nullptr, const NodeSizes* sizes = nullptr);
+
+// Local diffusion by matrix-free GMRES (FP32 inner cycles, FP64 residual
+// test), with the band LU for the nodes that miss the tolerance. Single
+// sub-step only. conv/iters (device, one int per node) report the outcome.
+struct GmresWork {
+    double* x;     // [nodes][nfull]
+    float* d32;    // [nodes][9][nfull]
+    float* pf;     // [nodes][3][nfull]
+    float* V;      // [nodes][25][nfull]
+    float* z;      // [nodes][2][nfull]
+    int* conv;     // [nodes]
+    int* iters;    // [nodes]
+};
+int launch_local_gmres(const verb_grid& g, double* psd, const LocalWork& w, const GmresWork& gw, double tol,
+                       int restart, int max_cycles, double inner_tol, int* status, hipStream_t stream,
+                       hipEvent_t* marks = nullptr);
 
 // Single-system LAPACK band routines on device buffers (one system).
 int launch_dgbtrf(int n, int kl, int ku, double* ab, int ldab, int64_t* ipiv, int* status, hipStream_t stream);

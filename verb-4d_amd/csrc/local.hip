@@ -19,6 +19,7 @@
 // so kl = ku = nV+1 as in the reference.
 #include "common.h"
 #include "band_lu.h"
+#include "gmres.h"
 
 namespace verb {
 namespace {
@@ -385,6 +386,18 @@ __global__ void scatter_psd(const verb_grid g, const double* __restrict__ rhs, d
     psd[(size_t)node * nfull + (size_t)ix * g.nK + k0 + iy] = rhs[(size_t)node * nfull + iy * nx + ix];
 }
 
+// psd slice (K-fast) -> energy-fast vector, the inverse of scatter_psd
+__global__ void gather_psd(const verb_grid g, const double* __restrict__ psd, double* __restrict__ x) {
+    const int node = blockIdx.y;
+    const int iR = node % g.nR;
+    if (node_skipped(g, iR)) return;
+    const int nx = g.nV, ny = g.nK, n = nx * ny, nfull = n;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int ix = p / ny, iy = p % ny;
+    x[(size_t)node * nfull + iy * nx + ix] = psd[(size_t)node * nfull + (size_t)ix * g.nK + iy];
+}
+
 int solve_chunk(int n, int width) {
     int dev = 0, max_lds = 0;
     hipGetDevice(&dev);
@@ -497,6 +510,54 @@ int launch_dgbtrs(int n, int kl, int ku, const double* ab, int ldab, const int64
     hipFuncSetAttribute((const void*)band_solve_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)slds);
     hipLaunchKernelGGL(band_solve_kernel, dim3(1), dim3(kSolveThreads), slds, stream, ab, ipiv, (size_t)0, n, kl, ku,
                        ldab, (const double*)nullptr, (const double*)nullptr, b, 1, chunk, 0, 1, (const int*)nullptr);
+    return hipGetLastError() == hipSuccess ? VERB_OK : VERB_ERR_HIP;
+}
+
+
+int launch_local_gmres(const verb_grid& g, double* psd, const LocalWork& w, const GmresWork& gw, double tol,
+                       int restart, int max_cycles, double inner_tol, int* status, hipStream_t stream,
+                       hipEvent_t* marks) {
+    if (g.nV < 3 || g.nK < 3) return VERB_ERR_ARG;
+    const int nx = g.nV, ny = g.nK, n = nx * ny;
+    const int kl = nx + 1, ku = nx + 1, ldab = 2 * kl + ku + 1;
+    const int nodes = g.nP * g.nR;
+    const int num_substeps = max(1, (int)(g.dt / g.sub_dt));
+    if (num_substeps != 1) return VERB_ERR_ARG;
+    if (kl > 112) return VERB_ERR_ARG;  // the fallback needs the register-window LU
+    const double dt_sub = g.dt / num_substeps;
+    dim3 agrid((n + kAsmThreads - 1) / kAsmThreads, nodes);
+    if (marks) hipEventRecord(marks[0], stream);
+    hipLaunchKernelGGL(local_assemble, agrid, dim3(kAsmThreads), 0, stream, g, psd, w.ab, w.b0, w.c0, w.rhs, w.diag,
+                       dt_sub, 0, 1, status, (const int*)nullptr);
+    hipLaunchKernelGGL(gather_psd, agrid, dim3(kAsmThreads), 0, stream, g, psd, gw.x);
+    if (marks) {
+        hipEventRecord(marks[1], stream);
+        hipEventRecord(marks[2], stream);
+    }
+    GmArgs ga;
+    ga.diag = w.diag; ga.rhs = w.rhs; ga.x = gw.x; ga.d32 = gw.d32; ga.pf = gw.pf; ga.V = gw.V; ga.z = gw.z;
+    ga.conv = gw.conv; ga.iters = gw.iters;
+    ga.nx = nx; ga.ny = ny; ga.nfull = n; ga.m = restart; ga.max_cycles = max_cycles;
+    ga.tol = tol; ga.inner_tol = (float)inner_tol; ga.nR = g.nR; ga.include_boundary = g.include_boundary;
+    int rc = launch_gmres(ga, nodes, stream);
+    if (rc) return rc;
+    if (marks) {
+        hipEventRecord(marks[3], stream);
+        hipEventRecord(marks[4], stream);
+    }
+    // nodes that missed the tolerance: the plain band-LU path, into the same x
+    LuArgs la;
+    la.diag = w.diag; la.rhs = w.rhs; la.y = w.y; la.ab = w.ab; la.ipiv = w.ipiv; la.uext = w.uext;
+    la.n = n; la.nx = nx; la.write_L = 0; la.status = status; la.nR = g.nR;
+    la.include_boundary = g.include_boundary;
+    la.nn = nullptr; la.order = nullptr; la.done = gw.conv;
+    rc = launch_band_lu(la, nodes, stream);
+    if (rc) return rc;
+    rc = launch_backsolve(w.ab, w.y, w.uext, gw.x, n, kl, ku, ldab, nodes, g.nR, g.include_boundary, stream, nullptr,
+                          nullptr, gw.conv);
+    if (rc) return rc;
+    hipLaunchKernelGGL(scatter_psd, agrid, dim3(kAsmThreads), 0, stream, g, gw.x, psd, (const int*)nullptr);
+    if (marks) hipEventRecord(marks[5], stream);
     return hipGetLastError() == hipSuccess ? VERB_OK : VERB_ERR_HIP;
 }
 

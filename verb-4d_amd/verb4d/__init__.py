@@ -45,6 +45,23 @@ class VerbError(RuntimeError):
         self.status = status
 
 
+class GmresParams(ctypes.Structure):
+    """ctypes mirror of ``verb_gmres_params`` (include/verb4d_hip.h)."""
+
+    _fields_ = (("tol", ctypes.c_double), ("restart", ctypes.c_int), ("max_cycles", ctypes.c_int),
+                ("inner_tol", ctypes.c_double))
+
+
+class GmresStats(ctypes.Structure):
+    """ctypes mirror of ``verb_gmres_stats``."""
+
+    _fields_ = (("nodes", ctypes.c_int), ("converged", ctypes.c_int), ("fallback", ctypes.c_int),
+                ("iterations", ctypes.c_longlong), ("max_iterations", ctypes.c_int))
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class VerbGrid(ctypes.Structure):
     """ctypes mirror of ``verb_grid`` (include/verb4d_hip.h)."""
 
@@ -118,6 +135,10 @@ _SIGNATURES = [
      [_P, ctypes.POINTER(VerbGrid), ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
       ctypes.c_void_p, ctypes.c_int]),
     ("verb_check_status", ctypes.c_int, [_P]),
+    ("verb_local_diffusion_gmres", ctypes.c_int,
+     [_P, ctypes.POINTER(VerbGrid), ctypes.c_void_p, ctypes.POINTER(GmresParams), ctypes.POINTER(GmresStats)]),
+    ("verb_step_gmres", ctypes.c_int,
+     [_P, ctypes.POINTER(VerbGrid), ctypes.c_void_p, ctypes.POINTER(GmresParams), ctypes.POINTER(GmresStats)]),
 ]
 
 
@@ -169,15 +190,15 @@ class Context:
     def check_status(self):
         return self.check(self._lib.verb_check_status(self.handle))
 
-    PHASES = ("convection", "radial", "local_assemble", "local_factor", "local_solve")
+    PHASES = ("convection", "radial", "local_assemble", "local_factor", "local_solve", "local_krylov")
 
     def set_timing(self, on=True):
         self.check(self._lib.verb_ctx_set_timing(self.handle, 1 if on else 0))
 
     def timings(self):
         """{phase: (total ms, launches)} since the last call (synchronises)."""
-        ms = (ctypes.c_double * 5)()
-        cnt = (ctypes.c_long * 5)()
+        ms = (ctypes.c_double * len(self.PHASES))()
+        cnt = (ctypes.c_long * len(self.PHASES))()
         self.check(self._lib.verb_ctx_get_timings(self.handle, ms, cnt))
         return {p: (ms[i], cnt[i]) for i, p in enumerate(self.PHASES)}
 

@@ -8,7 +8,7 @@ import ctypes
 
 import numpy as np
 
-from . import Context, VerbGrid, lib
+from . import Context, GmresParams, GmresStats, VerbGrid, lib
 
 
 class DeviceGrid:
@@ -99,6 +99,26 @@ class DeviceGrid:
         self.ctx.check(lib().verb_step_uneven(self.ctx.handle, ctypes.byref(self.grid), self.psd.data_ptr(),
                                               k.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), int(run_convection),
                                               int(run_radial), int(run_local)))
+
+    @staticmethod
+    def gmres_params(tol=1e-11, restart=20, max_cycles=8, inner_tol=1e-5):
+        return GmresParams(tol, restart, max_cycles, inner_tol)
+
+    def local_gmres(self, params=None, stats=True):
+        """Local diffusion by matrix-free GMRES with band-LU fallback; returns
+        the stats dict (synchronises) or None."""
+        p = params or self.gmres_params()
+        st = GmresStats() if stats else None
+        self.ctx.check(lib().verb_local_diffusion_gmres(self.ctx.handle, ctypes.byref(self.grid), self.psd.data_ptr(),
+                                                        ctypes.byref(p), ctypes.byref(st) if st else None))
+        return st.as_dict() if st else None
+
+    def step_gmres(self, params=None, stats=False):
+        p = params or self.gmres_params()
+        st = GmresStats() if stats else None
+        self.ctx.check(lib().verb_step_gmres(self.ctx.handle, ctypes.byref(self.grid), self.psd.data_ptr(),
+                                             ctypes.byref(p), ctypes.byref(st) if st else None))
+        return st.as_dict() if st else None
 
     def check_status(self):
         return self.ctx.check_status()
