@@ -142,7 +142,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=3.0, help="target seconds per operator sample")
     ap.add_argument("--cpu-threads", type=int, default=16)
-    ap.add_argument("--cached", action="store_true", help="also time steps that reuse cached LU factors")
+    ap.add_argument("--no-cached", action="store_true",
+                    help="skip timing steps that reuse cached LU factors (the reference's LU_CACHING mode)")
+    ap.add_argument("--no-gmres", action="store_true",
+                    help="skip timing steps whose local solve is the matrix-free GMRES path")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 512 MiB device-to-device copy three times (a known-size kernel against "
                          "which the FETCH_SIZE / WRITE_SIZE counters are scaled) and exit")
@@ -220,7 +223,10 @@ def main():
                 "frac": d["frac"], "traffic": profiled_traffic(dom), "kernel": dom}
 
     cached = None
-    if args.cached:
+    if not args.no_cached:
+        # LU caching on: factors from the first step are reused (the
+        # coefficients do not change between Kp updates)
+        dg.set_psd(w.psd0)
         dg.alloc_lu_cache()
         dg.step(use_cache=True, recompute_lu=True)
         ctx.synchronize()
@@ -230,7 +236,29 @@ def main():
         ev1.record(ext)
         ctx.synchronize()
         torch.cuda.synchronize()
-        cached = 1000.0 * args.steps / ev0.elapsed_time(ev1)
+        cached = {"steps_per_s": round(1000.0 * args.steps / ev0.elapsed_time(ev1), 4),
+                  "ms_per_step": round(ev0.elapsed_time(ev1) / args.steps, 4)}
+        dg.lu_cache = None
+        dg.ipiv_cache = None
+        torch.cuda.empty_cache()
+
+    gm = None
+    if not args.no_gmres:
+        # local solve by FP32 GMRES cycles with the FP64 residual test
+        # (max|b - Ax| <= 1e-11 max|b|) and band-LU fallback
+        dg.set_psd(w.psd0)
+        prm = DeviceGrid.gmres_params(tol=1e-11, restart=20, max_cycles=8)
+        st = dg.step_gmres(prm, stats=True)
+        ctx.synchronize()
+        ev0.record(ext)
+        for _ in range(args.steps):
+            dg.step_gmres(prm)
+        ev1.record(ext)
+        ctx.synchronize()
+        torch.cuda.synchronize()
+        gm = {"steps_per_s": round(1000.0 * args.steps / ev0.elapsed_time(ev1), 4),
+              "ms_per_step": round(ev0.elapsed_time(ev1) / args.steps, 4), "tol": 1e-11,
+              "first_step": st}
 
     base = None
     if not args.no_cpu_baseline:
@@ -260,7 +288,9 @@ def main():
         "phases": phase_out,
     }
     if cached is not None:
-        out["lu_cached_steps_per_s"] = round(cached, 4)
+        out["lu_cached"] = cached
+    if gm is not None:
+        out["gmres_local"] = gm
     if base:
         out["speedup_vs_cpu"] = round(value / base["value"], 2)
     print(json.dumps(out), flush=True)

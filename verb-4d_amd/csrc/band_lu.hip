@@ -533,12 +533,15 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgs a) {
 // (kl+ku+16) columns (the factorisation stores every U row up to kl+ku past
 // its diagonal, zeros included), with buffer loads issued one block ahead
 // into registers and written to LDS after the current block is solved; x
-// stays in LDS. Within a block: the part of each row right of the block is
-// reduced by 16 threads per row, then one wavefront solves the 16x16
-// triangle last row first, broadcasting each x with readlane.
+// is kept in LDS only over the band ahead of the current block (a ring of
+// kXRing entries; the solution streams to HBM), so two nodes fit in one CU.
+// Within a block: the part of each row right of the block is reduced by 16
+// threads per row, then one wavefront solves the 16x16 triangle last row
+// first, broadcasting each x with readlane.
 // ---------------------------------------------------------------------------
 constexpr int kBsThreads = 256;
 constexpr int kBsMaxI = 15;  // staged 16-column groups per thread: (kl+ku+16)/16 <= 15
+constexpr int kXRing = 256;  // x entries kept in LDS: >= kl+ku+16, a power of two
 
 __global__ __launch_bounds__(kBsThreads) void band_backsolve(const double* __restrict__ ab_all,
                                                              const double* __restrict__ y_all,
@@ -559,8 +562,8 @@ __global__ __launch_bounds__(kBsThreads) void band_backsolve(const double* __res
     const double* yv = y_all + (size_t)node * stride;
     double* xo = x_all + (size_t)node * stride;
     extern __shared__ double sh[];
-    double* x = sh;                    // n
-    double* stg = sh + n;              // 2 x [NB][WS]
+    double* x = sh;                    // ring of kXRing: x[i] at i % kXRing
+    double* stg = sh + kXRing;         // 2 x [NB][WS]
     double* part = stg + 2 * NB * WS;  // [NB][16] partial sums
     const int tid = threadIdx.x;
     const int ri = tid & 15, t = tid >> 4;
@@ -568,10 +571,13 @@ __global__ __launch_bounds__(kBsThreads) void band_backsolve(const double* __res
         __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(ab), (short)0, ldab * n * (int)sizeof(double), 0x00020000);
     const int nblk = (n + NB - 1) / NB;
 
-    double v[kBsMaxI];
+    double va[kBsMaxI], vb[kBsMaxI];
+    double ya = 0.0, yb = 0.0;
     // this thread's entries of block blk: row jb+ri, columns jb + t + 16i
-    auto issue = [&](int blk) {
+    // (and, for the 16 threads of the triangle, y of its row)
+    auto issue = [&](int blk, double (&v)[kBsMaxI], double& yr) {
         const int jb = NB * blk, r = jb + ri;
+        if (tid < NB) yr = (r < n) ? yv[r] : 0.0;
         const int hi = min(r + kv, n - 1);
 #pragma unroll
         for (int i = 0; i < kBsMaxI; i++) {
@@ -582,7 +588,7 @@ __global__ __launch_bounds__(kBsThreads) void band_backsolve(const double* __res
             v[i] = load_entry(rs, ok ? off : kOutside);
         }
     };
-    auto commit = [&](int buf) {
+    auto commit = [&](int buf, const double (&v)[kBsMaxI]) {
         double* st = stg + buf * NB * WS;
 #pragma unroll
         for (int i = 0; i < kBsMaxI; i++) {
@@ -590,23 +596,16 @@ __global__ __launch_bounds__(kBsThreads) void band_backsolve(const double* __res
             if ((i < nI) & (co < WS)) st[ri * WS + co] = v[i];
         }
     };
-
-    for (int i = tid; i < n; i += kBsThreads) x[i] = yv[i];
-    issue(nblk - 1);
-    commit((nblk - 1) & 1);
-    __syncthreads();
-    for (int blk = nblk - 1; blk >= 0; blk--) {
-        const int buf = blk & 1;
+    // one block: partial sums over the band right of it, then the triangle
+    auto solve_block = [&](int blk, double yr) {
         const int jb = NB * blk;
-        const double* st = stg + buf * NB * WS;
-        if (blk > 0) issue(blk - 1);
-        // right-of-block part: row ri, threads t = 0..15 stride over columns
+        const double* st = stg + (blk & 1) * NB * WS;
         {
             double sacc = 0.0;
             const int r = jb + ri;
             if (r < n) {
                 const int hi = min(r + kv, n - 1) - jb;
-                for (int co = NB + t; co <= hi; co += NB) sacc += st[ri * WS + co] * x[jb + co];
+                for (int co = NB + t; co <= hi; co += NB) sacc += st[ri * WS + co] * x[(jb + co) & (kXRing - 1)];
             }
             part[ri * 16 + t] = sacc;
         }
@@ -623,7 +622,7 @@ __global__ __launch_bounds__(kBsThreads) void band_backsolve(const double* __res
                 double sp = 0.0;
 #pragma unroll
                 for (int q = 0; q < 16; q++) sp += part[i * 16 + q];
-                b = x[r] - sp;
+                b = yr - sp;
 #pragma unroll
                 for (int k = 0; k < NB; k++) urow[k] = st[i * WS + k];
                 rd = 1.0 / st[i * WS + i];
@@ -632,24 +631,173 @@ __global__ __launch_bounds__(kBsThreads) void band_backsolve(const double* __res
             for (int k = NB - 1; k >= 0; k--) {
                 if (jb + k < n) {
                     const double xk = readlane_d(b * rd, k);
-                    if (tid == k) x[jb + k] = xk;
+                    if (tid == k) {
+                        x[(jb + k) & (kXRing - 1)] = xk;
+                        xo[jb + k] = xk;
+                    }
                     b = (i < k) ? fma(-urow[k], xk, b) : b;
                 }
             }
         }
-        if (blk > 0) commit((blk - 1) & 1);
+    };
+
+    // U blocks are requested two blocks ahead of their solve: while block blk
+    // is solved from LDS, block blk-1 waits in registers and block blk-2 is
+    // in flight
+    const int top = nblk - 1;
+    double ycur = 0.0;
+    issue(top, va, ycur);
+    commit(top & 1, va);
+    if (top >= 1) issue(top - 1, vb, yb);
+    __syncthreads();
+    for (int blk = top; blk >= 0; blk -= 2) {
+        // va is free, vb holds block blk-1
+        if (blk >= 2) issue(blk - 2, va, ya);
+        solve_block(blk, ycur);
+        ycur = yb;
+        if (blk >= 1) commit((blk - 1) & 1, vb);
+        __syncthreads();
+        if (blk < 1) break;
+        // vb is free, va holds block blk-2
+        if (blk >= 3) issue(blk - 3, vb, yb);
+        solve_block(blk - 1, ycur);
+        ycur = ya;
+        if (blk >= 2) commit((blk - 2) & 1, va);
         __syncthreads();
     }
-    for (int i = tid; i < n; i += kBsThreads) xo[i] = x[i];
+}
+
+// ---------------------------------------------------------------------------
+// y = L^-1 P b with factors in LAPACK band layout and dgbtf2-order
+// multipliers (the dgbtrs forward loop: for j < n-1, swap b(j) with
+// b(ipiv(j)), then b(j+1..j+lm) -= L(:,j) * b(j), skipped when b(j) = 0).
+// One wavefront per node. The 128 entries of b that the next 16 columns
+// touch live in two registers per lane (positions l and l+64 of the window
+// starting at column j0); interchanges and the pivot value go through
+// readlane, and each lane's multipliers for all 16 columns of a block
+// are loaded (column j's multipliers are contiguous, so the
+// lanes of one column load consecutive words), two blocks ahead of use. When B0 is given, the
+// right-hand side is formed first as rhs*B0 + C0 (src/MatrixSolver.cpp:318).
+// ---------------------------------------------------------------------------
+constexpr int kFwdBlock = 16;
+
+__global__ __launch_bounds__(64, 1) void band_forward(const double* __restrict__ ab_all,
+                                                   const int64_t* __restrict__ ipiv_all, int n, int kl, int ku,
+                                                   int ldab, const double* __restrict__ b0_all,
+                                                   const double* __restrict__ c0_all,
+                                                   const double* __restrict__ rhs_all, double* __restrict__ y_all,
+                                                   int nR, int include_boundary) {
+    const int node = blockIdx.x;
+    if (nR > 0 && !include_boundary && (node % nR == 0 || node % nR == nR - 1)) return;
+    const int kd = kl + ku;
+    const double* ab = ab_all + (size_t)node * ldab * n;
+    const int64_t* ipiv = ipiv_all + (size_t)node * n;
+    const double* rhs = rhs_all + (size_t)node * n;
+    const double* B0 = b0_all ? b0_all + (size_t)node * n : nullptr;
+    const double* C0 = c0_all ? c0_all + (size_t)node * n : nullptr;
+    double* y = y_all + (size_t)node * n;
+    const int l = threadIdx.x;
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(ab), (short)0, ldab * n * (int)sizeof(double), 0x00020000);
+    auto b_at = [&](int i) -> double {
+        if (i >= n) return 0.0;
+        if (!B0) return rhs[i];
+        return __dadd_rn(__dmul_rn(rhs[i], B0[i]), C0[i]);
+    };
+    double w0 = b_at(l), w1 = b_at(64 + l);
+    const int ncol = n - 1;  // columns with an elimination step
+    const int nblk = (ncol + kFwdBlock - 1) / kFwdBlock;
+    // multipliers of block blk for this lane: column j0+c, the rows that
+    // window positions l and 64+l are at (i = position - c - 1)
+    double La0[kFwdBlock], Lb0[kFwdBlock], La1[kFwdBlock], Lb1[kFwdBlock], La2[kFwdBlock], Lb2[kFwdBlock];
+    int pv0 = 0, pv1 = 0, pv2 = 0;
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+    // multipliers of block blk for this lane: column j0+c, the rows that
+    // window positions l and 64+l are at (i = position - c - 1)
+    auto fetch = [&](int blk, double (&La)[kFwdBlock], double (&Lb)[kFwdBlock], int& pv, double& fr) {
+        const int j0 = kFwdBlock * blk;
+        fr = (l >= 64 - kFwdBlock) ? b_at(j0 + kFwdBlock + 64 + l) : 0.0;  // enters the window after this block
+#pragma unroll
+        for (int c = 0; c < kFwdBlock; c++) {
+            const int j = j0 + c;
+            const int lm = min(kl, n - 1 - j);
+            const int i0 = l - c - 1, i1 = 63 + l - c;
+            const bool ok0 = (j < ncol) & (i0 >= 0) & (i0 < lm);
+            const bool ok1 = (j < ncol) & (i1 >= 0) & (i1 < lm);
+            const unsigned base = (unsigned)(j * ldab + kd + 1);
+            La[c] = load_entry(rs, ok0 ? (base + (unsigned)i0) * 8u : kOutside);
+            Lb[c] = load_entry(rs, ok1 ? (base + (unsigned)i1) * 8u : kOutside);
+        }
+        pv = (l < kFwdBlock && j0 + l < ncol) ? (int)ipiv[j0 + l] - 1 - j0 : l;
+    };
+    auto process = [&](int blk, const double (&La)[kFwdBlock], const double (&Lb)[kFwdBlock], int pv, double fresh) {
+        const int j0 = kFwdBlock * blk;
+#pragma unroll
+        for (int c = 0; c < kFwdBlock; c++) {
+            const int j = j0 + c;
+            if (j < ncol) {
+                const int lm = min(kl, n - 1 - j);
+                const int p = __builtin_amdgcn_readlane(pv, c);  // window position of row ipiv(j)
+                if (p != c) {
+                    const double vc = readlane_d(w0, c);
+                    const double vp = (p < 64) ? readlane_d(w0, p) : readlane_d(w1, p - 64);
+                    if (l == c) w0 = vp;
+                    if (p < 64) {
+                        if (l == p) w0 = vc;
+                    } else {
+                        if (l == p - 64) w1 = vc;
+                    }
+                }
+                const double bj = readlane_d(w0, c);
+                if (bj != 0.0) {
+                    const double t = -bj;
+                    // only rows j+1..j+lm change
+                    w0 = (l > c && l <= c + lm) ? fma(La[c], t, w0) : w0;
+                    w1 = (63 + l - c < lm) ? fma(Lb[c], t, w1) : w1;
+                }
+            }
+        }
+        // columns j0..j0+15 are final: out, and the window moves by 16
+        if (l < kFwdBlock && j0 + l < n) y[j0 + l] = w0;
+        const double s0 = __shfl(w0, (l + kFwdBlock) & 63);
+        const double s1 = __shfl(w1, (l + kFwdBlock) & 63);
+        w0 = (l < 64 - kFwdBlock) ? s0 : s1;
+        w1 = (l < 64 - kFwdBlock) ? s1 : fresh;
+    };
+    // multipliers are requested two blocks ahead of their use
+    if (nblk > 0) fetch(0, La0, Lb0, pv0, f0);
+    if (nblk > 1) fetch(1, La1, Lb1, pv1, f1);
+    for (int blk = 0; blk < nblk; blk += 3) {
+        if (blk + 2 < nblk) fetch(blk + 2, La2, Lb2, pv2, f2);
+        process(blk, La0, Lb0, pv0, f0);
+        if (blk + 1 >= nblk) break;
+        if (blk + 3 < nblk) fetch(blk + 3, La0, Lb0, pv0, f0);
+        process(blk + 1, La1, Lb1, pv1, f1);
+        if (blk + 2 >= nblk) break;
+        if (blk + 4 < nblk) fetch(blk + 4, La1, Lb1, pv1, f1);
+        process(blk + 2, La2, Lb2, pv2, f2);
+    }
+    const int j0 = kFwdBlock * nblk;
+    if (j0 + l < n) y[j0 + l] = w0;
+    if (j0 + 64 + l < n) y[j0 + 64 + l] = w1;
+}
+
+int launch_forward(const double* ab, const int64_t* ipiv, int n, int kl, int ku, int ldab, const double* b0,
+                   const double* c0, const double* rhs, double* y, int nodes, int nR, int include_boundary,
+                   hipStream_t stream) {
+    if (kl + kFwdBlock > 128 || (size_t)ldab * n * sizeof(double) >= 0x7ffffff8u) return VERB_ERR_ARG;
+    hipLaunchKernelGGL(band_forward, dim3(nodes), dim3(64), 0, stream, ab, ipiv, n, kl, ku, ldab, b0, c0, rhs, y, nR,
+                       include_boundary);
+    return hipGetLastError() == hipSuccess ? VERB_OK : VERB_ERR_HIP;
 }
 
 int launch_backsolve(const double* ab, const double* y, const int* uext, double* x, int n, int kl, int ku, int ldab,
                      int nodes, int nR, int include_boundary, hipStream_t stream, const int* nn, const int* order,
                      const int* done) {
     (void)uext;  // U rows are stored up to kl+ku past the diagonal, so extents are not needed
-    if (kl + ku + NB > NB * kBsMaxI) return VERB_ERR_ARG;
+    if (kl + ku + NB > NB * kBsMaxI || kl + ku + 2 * NB > kXRing) return VERB_ERR_ARG;
     const int WS = kl + ku + NB;
-    const size_t lds = sizeof(double) * ((size_t)n + 2 * NB * WS + NB * 16);
+    const size_t lds = sizeof(double) * ((size_t)kXRing + 2 * NB * WS + NB * 16);
     int dev = 0, max_lds = 0;
     hipGetDevice(&dev);
     hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);

@@ -42,6 +42,11 @@ int launch_band_lu(const LuArgs& a, int nodes, hipStream_t stream);
 int launch_backsolve(const double* ab, const double* y, const int* uext, double* x, int n, int kl, int ku, int ldab,
                      int nodes, int nR, int include_boundary, hipStream_t stream, const int* nn = nullptr,
                      const int* order = nullptr, const int* done = nullptr);
+// y = L^-1 P (B0*rhs + C0) (B0, C0 nullable: y = L^-1 P rhs) with LAPACK
+// band factors (dgbtf2-order multipliers); kl <= 112.
+int launch_forward(const double* ab, const int64_t* ipiv, int n, int kl, int ku, int ldab, const double* b0,
+                   const double* c0, const double* rhs, double* y, int nodes, int nR, int include_boundary,
+                   hipStream_t stream);
 // 0 = automatic (register-window LU where an instantiation covers kl),
 // 1 = force the one-column-per-step reference-order kernel.
 extern int local_lu_variant;

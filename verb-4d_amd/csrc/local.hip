@@ -484,10 +484,22 @@ int launch_local(const verb_grid& g, double* psd, const LocalWork& w, bool facto
         hipEventRecord(marks[3], stream);
         hipEventRecord(marks[4], stream);
     }
-    const size_t slds = sizeof(double) * ((size_t)n + (size_t)chunk * (kl + ku + 1));
-    hipFuncSetAttribute((const void*)band_solve_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)slds);
-    hipLaunchKernelGGL(band_solve_kernel, dim3(nodes), dim3(kSolveThreads), slds, stream, w.ab, w.ipiv, ab_stride,
-                       n, kl, ku, ldab, w.b0, w.c0, w.rhs, num_substeps, chunk, g.nR, g.include_boundary, nns);
+    if (window && !sizes) {
+        // per sub-step: y = L^-1 P (B0*x + C0), then x = U^-1 y
+        for (int s = 0; s < num_substeps; s++) {
+            int rc = launch_forward(w.ab, w.ipiv, n, kl, ku, ldab, w.b0, w.c0, w.rhs, w.y, nodes, g.nR,
+                                    g.include_boundary, stream);
+            if (rc) return rc;
+            rc = launch_backsolve(w.ab, w.y, w.uext, w.rhs, n, kl, ku, ldab, nodes, g.nR, g.include_boundary, stream);
+            if (rc) return rc;
+        }
+    } else {
+        const size_t slds = sizeof(double) * ((size_t)n + (size_t)chunk * (kl + ku + 1));
+        hipFuncSetAttribute((const void*)band_solve_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)slds);
+        hipLaunchKernelGGL(band_solve_kernel, dim3(nodes), dim3(kSolveThreads), slds, stream, w.ab, w.ipiv,
+                           ab_stride, n, kl, ku, ldab, w.b0, w.c0, w.rhs, num_substeps, chunk, g.nR,
+                           g.include_boundary, nns);
+    }
     hipLaunchKernelGGL(scatter_psd, agrid, dim3(kAsmThreads), 0, stream, g, w.rhs, psd, k0s);
     if (marks) hipEventRecord(marks[5], stream);
     return hipGetLastError() == hipSuccess ? VERB_OK : VERB_ERR_HIP;
