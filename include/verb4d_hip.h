@@ -213,6 +213,31 @@ int verb_local_diffusion_gmres(verb_ctx* ctx, const verb_grid* g, double* psd, c
 int verb_step_gmres(verb_ctx* ctx, const verb_grid* g, double* psd, const verb_gmres_params* params,
                     verb_gmres_stats* stats);
 
+/* ------------------------------------------------------------------------
+ * Launch shapes and the captured step. No counterpart in the reference (it
+ * has no device launches); they change speed, not results.
+ *   verb_set_tuning / verb_get_tuning: the ctx's launch shapes.
+ *   verb_autotune: times every convection tile size and every register-window
+ *     LU shape that covers nV (each with and without the persistent node
+ *     queue) on this grid and PSD, keeps the fastest in the ctx and reports
+ *     them; psd is restored before it returns.
+ *   verb_step_graph: the plain step (convection, radial, local with LU caching
+ *     off) of verb_step. The first call for a grid/psd/tuning runs it eagerly
+ *     and records it as a HIP graph; later calls replay the graph.
+ * ------------------------------------------------------------------------ */
+typedef struct {
+    int convection_tq;    /* (V,K) planes per convection workgroup, 0 = automatic */
+    int lu_rb, lu_cb;     /* register-window LU shape in 16-row / 16-column blocks, 0 = automatic */
+    int lu_persistent;    /* 1: one LU workgroup per CU takes nodes from a queue */
+    double convection_ms; /* verb_autotune: time of the chosen convection launch */
+    double local_ms;      /* verb_autotune: time of the chosen local-diffusion launches */
+} verb_tuning;
+
+int verb_set_tuning(verb_ctx* ctx, const verb_tuning* t);
+int verb_get_tuning(verb_ctx* ctx, verb_tuning* t);
+int verb_autotune(verb_ctx* ctx, const verb_grid* g, double* psd, verb_tuning* out);
+int verb_step_graph(verb_ctx* ctx, const verb_grid* g, double* psd);
+
 #ifdef __cplusplus
 }
 #endif

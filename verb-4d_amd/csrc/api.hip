@@ -12,7 +12,8 @@
 #include "gmres.h"
 
 namespace verb {
-int launch_convection_mode(const verb_grid& g, double* psd, int* status, hipStream_t stream, int single_outer);
+int launch_convection_mode(const verb_grid& g, double* psd, int* status, hipStream_t stream, int single_outer,
+                           int tq);
 }
 
 struct verb_ctx {
@@ -31,6 +32,16 @@ struct verb_ctx {
     std::vector<Mark> marks;
     double ms[VERB_NUM_PHASES] = {0};
     long counts[VERB_NUM_PHASES] = {0};
+    // launch shapes (verb_set_tuning / verb_autotune)
+    int conv_tq = 0;
+    int lu_variant = -1;
+    int lu_persistent = 0;
+    // captured plain step (verb_step_graph) and what it was captured for
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    verb_grid graph_grid{};
+    double* graph_psd = nullptr;
+    int graph_tune[3] = {0, -1, 0};
 };
 
 namespace {
@@ -98,6 +109,8 @@ struct LocalBuffers {
     int rc = 0;
 };
 
+constexpr int kSlotQueue = 48;  // persistent-LU node counter
+
 LocalBuffers local_buffers(verb_ctx* c, const verb_grid* g, double* lu_cache, int64_t* ipiv_cache, int base_slot) {
     LocalBuffers lb;
     const size_t n = (size_t)g->nV * g->nK;
@@ -116,7 +129,10 @@ LocalBuffers local_buffers(verb_ctx* c, const verb_grid* g, double* lu_cache, in
     lb.w.diag = (double*)scratch(c, base_slot + 5, 9 * n * nodes * sizeof(double));
     lb.w.y = (double*)scratch(c, base_slot + 6, n * nodes * sizeof(double));
     lb.w.uext = (int*)scratch(c, base_slot + 7, n * nodes * sizeof(int));
-    if (!lb.w.ab || !lb.w.ipiv || !lb.w.b0 || !lb.w.c0 || !lb.w.rhs || !lb.w.diag || !lb.w.y || !lb.w.uext)
+    lb.w.queue = c->lu_persistent ? (int*)scratch(c, kSlotQueue, sizeof(int)) : nullptr;
+    lb.w.lu_variant = c->lu_variant;
+    if (!lb.w.ab || !lb.w.ipiv || !lb.w.b0 || !lb.w.c0 || !lb.w.rhs || !lb.w.diag || !lb.w.y || !lb.w.uext ||
+        (c->lu_persistent && !lb.w.queue))
         lb.rc = set_err(c, VERB_ERR_NOMEM, "device allocation failed");
     return lb;
 }
@@ -175,6 +191,8 @@ int verb_ctx_create(verb_ctx** out) {
 int verb_ctx_destroy(verb_ctx* c) {
     if (!c) return VERB_ERR_ARG;
     hipStreamSynchronize(c->stream);
+    if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) hipGraphDestroy(c->graph);
     for (auto& b : c->bufs)
         if (b.first) hipFree(b.first);
     hipFree(c->status);
@@ -246,7 +264,7 @@ int verb_convection_batched(verb_ctx* c, const verb_grid* g, double* psd) {
     if (!c || !grid_ok(g) || !psd) return VERB_ERR_ARG;
     std::lock_guard<std::mutex> lk(c->mu);
     const int m = mark_begin(c, VERB_PHASE_CONVECTION);
-    int rc = verb::launch_convection(*g, psd, c->status, c->stream);
+    int rc = verb::launch_convection(*g, psd, c->status, c->stream, c->conv_tq);
     mark_end(c, m);
     return rc ? set_err(c, rc, "convection launch rejected") : VERB_OK;
 }
@@ -349,6 +367,7 @@ int verb_step_uneven(verb_ctx* c, const verb_grid* g, double* psd, const int* k0
 }
 
 constexpr int kSlotGmres = 60;  // scratch slots 60..66 for the GMRES workspace
+constexpr int kSlotTune = 70;   // the autotuner's copy of the PSD
 
 int verb_local_diffusion_gmres(verb_ctx* c, const verb_grid* g, double* psd, const verb_gmres_params* prm,
                                verb_gmres_stats* stats) {
@@ -413,6 +432,167 @@ int verb_step_gmres(verb_ctx* c, const verb_grid* g, double* psd, const verb_gmr
     if (!rc && g->nR >= 3) rc = verb_radial_diffusion_batched(c, g, psd);
     if (!rc && g->nV >= 3 && g->nK >= 3) rc = verb_local_diffusion_gmres(c, g, psd, prm, stats);
     return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Launch shapes, autotuning and the captured step
+// ---------------------------------------------------------------------------
+int verb_set_tuning(verb_ctx* c, const verb_tuning* t) {
+    if (!c || !t) return VERB_ERR_ARG;
+    int variant = -1;
+    if (t->lu_rb > 0 || t->lu_cb > 0) {
+        for (int i = 0; i < verb::lu_variant_count(); i++) {
+            int rb = 0, cb = 0;
+            verb::lu_variant_shape(i, 0, &rb, &cb);
+            if (rb == t->lu_rb && cb == t->lu_cb) variant = i;
+        }
+        if (variant < 0) return set_err(c, VERB_ERR_ARG, "no register-window LU of that shape");
+    }
+    if (t->convection_tq < 0 || t->convection_tq > 64) return set_err(c, VERB_ERR_ARG, "convection_tq out of range");
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->conv_tq = t->convection_tq;
+    c->lu_variant = variant;
+    c->lu_persistent = t->lu_persistent ? 1 : 0;
+    return VERB_OK;
+}
+
+int verb_get_tuning(verb_ctx* c, verb_tuning* t) {
+    if (!c || !t) return VERB_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    t->convection_tq = c->conv_tq;
+    t->lu_rb = t->lu_cb = 0;
+    if (c->lu_variant >= 0) verb::lu_variant_shape(c->lu_variant, 0, &t->lu_rb, &t->lu_cb);
+    t->lu_persistent = c->lu_persistent;
+    t->convection_ms = t->local_ms = 0.0;
+    return VERB_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Milliseconds of one call of fn on the ctx stream, psd restored from `save`
+// first (the restore is not timed). Negative when fn rejects the launch.
+template <class F>
+double time_once(verb_ctx* c, double* psd, const double* save, size_t bytes, F fn) {
+    if (hipMemcpyAsync(psd, save, bytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess) return -1.0;
+    hipEvent_t a = new_event(), b = new_event();
+    hipEventRecord(a, c->stream);
+    const int rc = fn();
+    hipEventRecord(b, c->stream);
+    hipEventSynchronize(b);
+    float ms = -1.0f;
+    if (rc == VERB_OK) hipEventElapsedTime(&ms, a, b);
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    return rc == VERB_OK ? (double)ms : -1.0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int verb_autotune(verb_ctx* c, const verb_grid* g, double* psd, verb_tuning* out) {
+    if (!c || !grid_ok(g) || !psd) return VERB_ERR_ARG;
+    const size_t bytes = (size_t)g->nP * g->nR * g->nV * g->nK * sizeof(double);
+    double* save = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(c->mu);
+        save = (double*)scratch(c, kSlotTune, bytes);
+        if (!save) return set_err(c, VERB_ERR_NOMEM, "device allocation failed");
+        HIPCHK(c, hipMemcpyAsync(save, psd, bytes, hipMemcpyDeviceToDevice, c->stream));
+    }
+    const bool saved_timing = c->timing;
+    c->timing = false;
+    verb_tuning best{};
+    // convection: planes per workgroup
+    best.convection_ms = -1.0;
+    if (3 < g->nP || 3 < g->nR) {
+        for (int tq : {1, 2, 4, 8}) {
+            c->conv_tq = tq;
+            time_once(c, psd, save, bytes, [&] { return verb_convection_batched(c, g, psd); });  // warm-up
+            const double ms = time_once(c, psd, save, bytes, [&] { return verb_convection_batched(c, g, psd); });
+            if (ms >= 0.0 && (best.convection_ms < 0.0 || ms < best.convection_ms)) {
+                best.convection_ms = ms;
+                best.convection_tq = tq;
+            }
+        }
+    }
+    c->conv_tq = best.convection_tq;
+    // local diffusion: register-window shape and persistent dispatch
+    best.local_ms = -1.0;
+    int best_variant = -1, best_pers = 0;
+    const int kl = g->nV + 1;
+    if (g->nV >= 3 && g->nK >= 3) {
+        for (int v = 0; v < verb::lu_variant_count(); v++) {
+            if (!verb::lu_variant_shape(v, kl, nullptr, nullptr)) continue;
+            for (int pers = 0; pers <= 1; pers++) {
+                c->lu_variant = v;
+                c->lu_persistent = pers;
+                auto run = [&] { return verb_local_diffusion_batched(c, g, psd, nullptr, nullptr, 1); };
+                time_once(c, psd, save, bytes, run);  // warm-up (workspace allocation)
+                const double ms = time_once(c, psd, save, bytes, run);
+                if (ms >= 0.0 && (best.local_ms < 0.0 || ms < best.local_ms)) {
+                    best.local_ms = ms;
+                    best_variant = v;
+                    best_pers = pers;
+                }
+            }
+        }
+    }
+    c->lu_variant = best_variant;
+    c->lu_persistent = best_pers;
+    if (best_variant >= 0) verb::lu_variant_shape(best_variant, kl, &best.lu_rb, &best.lu_cb);
+    best.lu_persistent = best_pers;
+    c->timing = saved_timing;
+    // the trials may have raised device status (e.g. a zero pivot): report it
+    int rc = verb_check_status(c);
+    HIPCHK(c, hipMemcpyAsync(psd, save, bytes, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (out) *out = best;
+    return rc;
+}
+
+int verb_step_graph(verb_ctx* c, const verb_grid* g, double* psd) {
+    if (!c || !grid_ok(g) || !psd) return VERB_ERR_ARG;
+    const int tune[3] = {c->conv_tq, c->lu_variant, c->lu_persistent};
+    const bool match = c->graph_exec && std::memcmp(&c->graph_grid, g, sizeof(verb_grid)) == 0 &&
+                       c->graph_psd == psd && std::memcmp(c->graph_tune, tune, sizeof(tune)) == 0;
+    if (match) {
+        HIPCHK(c, hipGraphLaunch(c->graph_exec, c->stream));
+        return VERB_OK;
+    }
+    // A new shape: this call runs the step eagerly (which also allocates the
+    // workspace outside any capture), then records the same launch sequence
+    // into a graph for the following calls.
+    int rc = verb_step(c, g, psd, 1, 1, 1, nullptr, nullptr, 1);
+    if (rc) return rc;
+    if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) hipGraphDestroy(c->graph);
+    c->graph_exec = nullptr;
+    c->graph = nullptr;
+    const bool saved_timing = c->timing;
+    c->timing = false;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    rc = verb_step(c, g, psd, 1, 1, 1, nullptr, nullptr, 1);
+    hipGraph_t gr = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &gr);
+    c->timing = saved_timing;
+    if (rc || e != hipSuccess || !gr) {
+        if (gr) hipGraphDestroy(gr);
+        return rc ? rc : set_err(c, VERB_ERR_HIP, "step capture failed");
+    }
+    hipGraphExec_t ex = nullptr;
+    if (hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0) != hipSuccess) {
+        hipGraphDestroy(gr);
+        return set_err(c, VERB_ERR_HIP, "step graph instantiation failed");
+    }
+    c->graph = gr;
+    c->graph_exec = ex;
+    c->graph_grid = *g;
+    c->graph_psd = psd;
+    std::memcpy(c->graph_tune, tune, sizeof(tune));
+    return VERB_OK;
 }
 
 int verb_step(verb_ctx* c, const verb_grid* g, double* psd, int run_convection, int run_radial, int run_local,
@@ -562,7 +742,7 @@ static int convection_call(verb_ctx* c, double* psd, const double* P, const doub
     g.Pl_type = Pl_type; g.Pu_type = Pu_type; g.Rl_type = Rl_type; g.Ru_type = Ru_type;
     g.dt = dt;
     if ((rc = reset_status(c))) return rc;
-    rc = verb::launch_convection_mode(g, dpsd, c->status, c->stream, single_outer);
+    rc = verb::launch_convection_mode(g, dpsd, c->status, c->stream, single_outer, 0);
     if (rc) return set_err(c, rc, "convection launch rejected");
     rc = finish(c);
     reset_status(c);

@@ -17,6 +17,8 @@ struct LuArgs {
     const int* nn;             // per-node unknowns (null: n for every node); buffers keep stride n
     const int* order;          // dispatch order of the nodes (null: identity)
     const int* done = nullptr; // nodes with done[node] != 0 are left alone (null: none)
+    int* queue = nullptr;      // persistent mode: workgroups take dispatch slots 0..qnodes-1 from this counter (zeroed)
+    int qnodes = 0;
 };
 
 }  // namespace verb

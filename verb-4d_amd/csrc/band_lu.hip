@@ -164,9 +164,12 @@ struct Lds {
     int jp[NB];
     int uext[NB];                            // U row extents of the panel rows
     int ju, info;
+    int qnode;                               // persistent mode: current dispatch slot
 };
 
 }  // namespace
+
+__global__ void zero_counter(int* c) { *c = 0; }
 
 template <int RB, int CB>
 __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgs a) {
@@ -176,9 +179,23 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgs a) {
     extern __shared__ double lds_raw[];
     L& S = *reinterpret_cast<L*>(lds_raw);
 
-    const int node = a.order ? a.order[blockIdx.x] : blockIdx.x;
-    if (a.nR > 0 && !a.include_boundary && (node % a.nR == 0 || node % a.nR == a.nR - 1)) return;
-    if (a.done && a.done[node]) return;
+    const int tid = threadIdx.x;
+    for (int it = 0;; it++) {
+    int node;
+    if (a.queue) {
+        // persistent mode: the next dispatch slot from the node queue
+        if (tid == 0) S.qnode = atomicAdd(a.queue, 1);
+        __syncthreads();
+        const int k = S.qnode;
+        __syncthreads();
+        if (k >= a.qnodes) break;
+        node = a.order ? a.order[k] : k;
+    } else {
+        if (it > 0) break;
+        node = a.order ? a.order[blockIdx.x] : blockIdx.x;
+    }
+    if (a.nR > 0 && !a.include_boundary && (node % a.nR == 0 || node % a.nR == a.nR - 1)) continue;
+    if (a.done && a.done[node]) continue;
     const int stride = a.n;
     const int n = a.nn ? a.nn[node] : a.n, nx = a.nx;
     const int kl = nx + 1, ku = nx + 1, kv = kl + ku, ldab = 2 * kl + ku + 1;
@@ -189,7 +206,6 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgs a) {
     int64_t* ipiv = a.ipiv + (size_t)node * stride;
     int* uext = a.uext ? a.uext + (size_t)node * stride : nullptr;
 
-    const int tid = threadIdx.x;
     // wavefront index as a uniform value: per-wavefront branches below are
     // scalar, so the register allocator sees them as exclusive
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -526,6 +542,8 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgs a) {
         __syncthreads();
     }
     if (tid == 0 && S.info != 0) raise_pivot(a.status, S.info, node);
+    __syncthreads();  // the LDS state is reused by the next node
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -817,21 +835,55 @@ template <int RB, int CB>
 int launch_band_lu_t(const LuArgs& a, int nodes, hipStream_t stream) {
     const size_t lds = band_lu_lds_bytes<RB, CB>();
     hipFuncSetAttribute((const void*)band_lu_window<RB, CB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL((band_lu_window<RB, CB>), dim3(nodes), dim3(RB * 64), lds, stream, a);
+    int grid = nodes;
+    if (a.queue) {
+        int dev = 0, cus = 0;
+        hipGetDevice(&dev);
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        grid = min(nodes, max(1, cus));
+        hipLaunchKernelGGL(zero_counter, dim3(1), dim3(1), 0, stream, a.queue);
+    }
+    hipLaunchKernelGGL((band_lu_window<RB, CB>), dim3(grid), dim3(RB * 64), lds, stream, a);
     return hipGetLastError() == hipSuccess ? VERB_OK : VERB_ERR_HIP;
 }
 
-// Returns VERB_ERR_ARG when no register-window instantiation covers kl.
-int launch_band_lu(const LuArgs& a, int nodes, hipStream_t stream) {
+namespace {
+
+struct LuVariant {
+    int rb, cb;
+    int (*launch)(const LuArgs&, int, hipStream_t);
+};
+
+// Register-window instantiations: RB*16 >= kl+16 rows, CB*16 >= 2kl+16 columns.
+const LuVariant kLuVariants[] = {
+    {3, 5, launch_band_lu_t<3, 5>},   {4, 7, launch_band_lu_t<4, 7>},   {5, 9, launch_band_lu_t<5, 9>},
+    {6, 11, launch_band_lu_t<6, 11>}, {7, 13, launch_band_lu_t<7, 13>}, {8, 14, launch_band_lu_t<8, 14>},
+    {8, 15, launch_band_lu_t<8, 15>},
+};
+
+bool lu_variant_fits(const LuVariant& v, int kl) { return NB * v.rb >= kl + NB && NB * v.cb >= 2 * kl + NB; }
+
+}  // namespace
+
+int lu_variant_count() { return (int)(sizeof(kLuVariants) / sizeof(kLuVariants[0])); }
+
+int lu_variant_shape(int i, int kl, int* rb, int* cb) {
+    if (i < 0 || i >= lu_variant_count()) return 0;
+    if (rb) *rb = kLuVariants[i].rb;
+    if (cb) *cb = kLuVariants[i].cb;
+    return lu_variant_fits(kLuVariants[i], kl) ? 1 : 0;
+}
+
+// variant < 0: the smallest instantiation that covers kl. Returns
+// VERB_ERR_ARG when no register-window instantiation covers kl.
+int launch_band_lu(const LuArgs& a, int nodes, hipStream_t stream, int variant) {
     const int kl = a.nx + 1;
-    // RB*16 >= kl+16, CB*16 >= 2kl+16
-    if (kl <= 32) return launch_band_lu_t<3, 5>(a, nodes, stream);
-    if (kl <= 48) return launch_band_lu_t<4, 7>(a, nodes, stream);
-    if (kl <= 64) return launch_band_lu_t<5, 9>(a, nodes, stream);
-    if (kl <= 80) return launch_band_lu_t<6, 11>(a, nodes, stream);
-    if (kl <= 96) return launch_band_lu_t<7, 13>(a, nodes, stream);
-    if (kl <= 104) return launch_band_lu_t<8, 14>(a, nodes, stream);
-    if (kl <= 112) return launch_band_lu_t<8, 15>(a, nodes, stream);
+    if (variant >= 0) {
+        if (variant >= lu_variant_count() || !lu_variant_fits(kLuVariants[variant], kl)) return VERB_ERR_ARG;
+        return kLuVariants[variant].launch(a, nodes, stream);
+    }
+    for (const LuVariant& v : kLuVariants)
+        if (lu_variant_fits(v, kl)) return v.launch(a, nodes, stream);
     return VERB_ERR_ARG;
 }
 

@@ -24,7 +24,8 @@ __device__ inline void raise_pivot(int* status, int col1, int node) {
 
 // Launchers (defined in the .hip files). All asynchronous on `stream`.
 int launch_radial(const verb_grid& g, double* psd, int* status, hipStream_t stream);
-int launch_convection(const verb_grid& g, double* psd, int* status, hipStream_t stream);
+// tq: (V,K) planes per workgroup (0: the largest of 4, 2, 1 that fits in LDS)
+int launch_convection(const verb_grid& g, double* psd, int* status, hipStream_t stream, int tq = 0);
 
 struct LocalWork {
     double* ab;       // band factors, per node (2kl+ku+1)*n doubles (LAPACK layout)
@@ -35,10 +36,16 @@ struct LocalWork {
     double* diag;     // per node 9n: the assembled operator (register-window LU path)
     double* y;        // per node n: forward-eliminated right-hand side
     int* uext;        // per node n: last nonzero column of each U row
+    int* queue = nullptr;  // device counter for the persistent LU launch (null: one workgroup per node)
+    int lu_variant = -1;   // register-window instantiation (-1: smallest that covers kl)
 };
 
 struct LuArgs;
-int launch_band_lu(const LuArgs& a, int nodes, hipStream_t stream);
+int launch_band_lu(const LuArgs& a, int nodes, hipStream_t stream, int variant = -1);
+// register-window instantiations: count, and (rb, cb) of entry i; returns
+// whether entry i covers kl
+int lu_variant_count();
+int lu_variant_shape(int i, int kl, int* rb, int* cb);
 int launch_backsolve(const double* ab, const double* y, const int* uext, double* x, int n, int kl, int ku, int ldab,
                      int nodes, int nR, int include_boundary, hipStream_t stream, const int* nn = nullptr,
                      const int* order = nullptr, const int* done = nullptr);

@@ -560,7 +560,8 @@ size_t convection_lds_bytes(int nP, int nR, int TQ) {
 
 int convection_tq = 0;  // 0 = choose automatically
 
-int launch_convection_mode(const verb_grid& g, double* psd, int* status, hipStream_t stream, int single_outer) {
+int launch_convection_mode(const verb_grid& g, double* psd, int* status, hipStream_t stream, int single_outer,
+                           int tq) {
     if (g.nP < 1 || g.nR < 1) return VERB_ERR_ARG;
     if ((g.nP >= 3 && g.nP < 6) || (g.nR >= 3 && g.nR < 6)) return VERB_ERR_ARG;  // stencil needs 6 points
     if ((g.Pl_type == VERB_BC_PERIODIC) != (g.Pu_type == VERB_BC_PERIODIC)) return VERB_ERR_ARG;
@@ -571,7 +572,7 @@ int launch_convection_mode(const verb_grid& g, double* psd, int* status, hipStre
     hipGetDevice(&dev);
     int max_lds = 0;
     hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-    int TQ = convection_tq;
+    int TQ = tq > 0 ? tq : convection_tq;
     if (TQ <= 0) {
         TQ = 4;
         while (TQ > 1 && convection_lds_bytes(g.nP, g.nR, TQ) > (size_t)max_lds) TQ /= 2;
@@ -588,8 +589,8 @@ int launch_convection_mode(const verb_grid& g, double* psd, int* status, hipStre
     return hipGetLastError() == hipSuccess ? VERB_OK : VERB_ERR_HIP;
 }
 
-int launch_convection(const verb_grid& g, double* psd, int* status, hipStream_t stream) {
-    return launch_convection_mode(g, psd, status, stream, 0);
+int launch_convection(const verb_grid& g, double* psd, int* status, hipStream_t stream, int tq) {
+    return launch_convection_mode(g, psd, status, stream, 0, tq);
 }
 
 }  // namespace verb

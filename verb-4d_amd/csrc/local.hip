@@ -440,7 +440,8 @@ int launch_local(const verb_grid& g, double* psd, const LocalWork& w, bool facto
         la.n = n; la.nx = nx; la.write_L = 0; la.status = status; la.nR = g.nR;
         la.include_boundary = g.include_boundary;
         la.nn = nns; la.order = order;
-        int rc = launch_band_lu(la, nodes, stream);
+        la.queue = w.queue; la.qnodes = nodes;
+        int rc = launch_band_lu(la, nodes, stream, w.lu_variant);
         if (rc) return rc;
         if (marks) {
             hipEventRecord(marks[3], stream);
@@ -473,7 +474,8 @@ int launch_local(const verb_grid& g, double* psd, const LocalWork& w, bool facto
         la.n = n; la.nx = nx; la.write_L = 1; la.status = status; la.nR = g.nR;
         la.include_boundary = g.include_boundary;
         la.nn = nns; la.order = order;
-        int rc = launch_band_lu(la, nodes, stream);
+        la.queue = w.queue; la.qnodes = nodes;
+        int rc = launch_band_lu(la, nodes, stream, w.lu_variant);
         if (rc) return rc;
     } else if (factor) {
         const size_t lds = sizeof(double) * (size_t)(kl + 1 + kl + ku + 1);
@@ -563,7 +565,8 @@ int launch_local_gmres(const verb_grid& g, double* psd, const LocalWork& w, cons
     la.n = n; la.nx = nx; la.write_L = 0; la.status = status; la.nR = g.nR;
     la.include_boundary = g.include_boundary;
     la.nn = nullptr; la.order = nullptr; la.done = gw.conv;
-    rc = launch_band_lu(la, nodes, stream);
+    la.queue = w.queue; la.qnodes = nodes;
+    rc = launch_band_lu(la, nodes, stream, w.lu_variant);
     if (rc) return rc;
     rc = launch_backsolve(w.ab, w.y, w.uext, gw.x, n, kl, ku, ldab, nodes, g.nR, g.include_boundary, stream, nullptr,
                           nullptr, gw.conv);

@@ -62,6 +62,16 @@ class GmresStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class Tuning(ctypes.Structure):
+    """ctypes mirror of ``verb_tuning``."""
+
+    _fields_ = (("convection_tq", ctypes.c_int), ("lu_rb", ctypes.c_int), ("lu_cb", ctypes.c_int),
+                ("lu_persistent", ctypes.c_int), ("convection_ms", ctypes.c_double), ("local_ms", ctypes.c_double))
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class VerbGrid(ctypes.Structure):
     """ctypes mirror of ``verb_grid`` (include/verb4d_hip.h)."""
 
@@ -139,6 +149,10 @@ _SIGNATURES = [
      [_P, ctypes.POINTER(VerbGrid), ctypes.c_void_p, ctypes.POINTER(GmresParams), ctypes.POINTER(GmresStats)]),
     ("verb_step_gmres", ctypes.c_int,
      [_P, ctypes.POINTER(VerbGrid), ctypes.c_void_p, ctypes.POINTER(GmresParams), ctypes.POINTER(GmresStats)]),
+    ("verb_set_tuning", ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),
+    ("verb_get_tuning", ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),
+    ("verb_autotune", ctypes.c_int, [_P, ctypes.POINTER(VerbGrid), ctypes.c_void_p, ctypes.POINTER(Tuning)]),
+    ("verb_step_graph", ctypes.c_int, [_P, ctypes.POINTER(VerbGrid), ctypes.c_void_p]),
 ]
 
 

@@ -8,7 +8,7 @@ import ctypes
 
 import numpy as np
 
-from . import Context, GmresParams, GmresStats, VerbGrid, lib
+from . import Context, GmresParams, GmresStats, Tuning, VerbGrid, lib
 
 
 class DeviceGrid:
@@ -119,6 +119,23 @@ class DeviceGrid:
         self.ctx.check(lib().verb_step_gmres(self.ctx.handle, ctypes.byref(self.grid), self.psd.data_ptr(),
                                              ctypes.byref(p), ctypes.byref(st) if st else None))
         return st.as_dict() if st else None
+
+    def autotune(self):
+        """Times the launch shapes on this grid and keeps the fastest in the
+        context; returns them as a dict (psd is left unchanged)."""
+        t = Tuning()
+        self.ctx.check(lib().verb_autotune(self.ctx.handle, ctypes.byref(self.grid), self.psd.data_ptr(),
+                                           ctypes.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, convection_tq=0, lu_rb=0, lu_cb=0, lu_persistent=0):
+        t = Tuning(convection_tq, lu_rb, lu_cb, lu_persistent, 0.0, 0.0)
+        self.ctx.check(lib().verb_set_tuning(self.ctx.handle, ctypes.byref(t)))
+
+    def step_graph(self):
+        """The plain step through a captured HIP graph (captured on the first
+        call for this grid, replayed afterwards)."""
+        self.ctx.check(lib().verb_step_graph(self.ctx.handle, ctypes.byref(self.grid), self.psd.data_ptr()))
 
     def check_status(self):
         return self.ctx.check_status()
