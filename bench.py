@@ -146,6 +146,9 @@ def main():
                     help="skip timing steps that reuse cached LU factors (the reference's LU_CACHING mode)")
     ap.add_argument("--no-gmres", action="store_true",
                     help="skip timing steps whose local solve is the matrix-free GMRES path")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="time eagerly launched steps instead of the captured HIP graph of the step")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 512 MiB device-to-device copy three times (a known-size kernel against "
                          "which the FETCH_SIZE / WRITE_SIZE counters are scaled) and exit")
@@ -176,6 +179,10 @@ def main():
     dg.check_status()
     torch.cuda.synchronize()
     ctx.synchronize()
+    tuning = None
+    if not args.no_autotune:
+        tuning = dg.autotune()  # launch shapes only; psd is restored
+        print("bench: autotuned %s" % tuning, file=sys.stderr, flush=True)
 
     ext = torch.cuda.ExternalStream(ctx.stream)
     ev0 = torch.cuda.Event(enable_timing=True)
@@ -197,8 +204,29 @@ def main():
     dg.check_status()
     ms_events = ev0.elapsed_time(ev1)
     ms_per_step = ms_events / args.steps
-    print("bench: %d steps, %.3f ms/step (events), wall %.3f s" % (args.steps, ms_per_step, wall),
+    print("bench: %d eager steps with phase events, %.3f ms/step, wall %.3f s" % (args.steps, ms_per_step, wall),
           file=sys.stderr, flush=True)
+    eager_ms_per_step = ms_per_step
+    if not args.no_graph:
+        # the same step replayed from its captured HIP graph: the headline
+        dg.step_graph()  # eager step + capture
+        for _ in range(max(1, args.warmup)):
+            dg.step_graph()
+        dg.check_status()
+        ctx.synchronize()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ev0.record(ext)
+        for _ in range(args.steps):
+            dg.step_graph()
+        ev1.record(ext)
+        ctx.synchronize()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        dg.check_status()
+        ms_per_step = ev0.elapsed_time(ev1) / args.steps
+        print("bench: %d graph-replayed steps, %.3f ms/step, wall %.3f s" % (args.steps, ms_per_step, wall),
+              file=sys.stderr, flush=True)
 
     work = phase_work(w)
     phase_out = {}
@@ -282,10 +310,13 @@ def main():
         "data": "synthetic",
         "config": {"workload": "production grid 29x24x101x91 (R x MLT x E x alpha), 696 band systems of 9191 "
                                "unknowns (kl=ku=102), Kp=2, dt=300 s, LU caching off",
-                   "grid": [w.nR, w.nP, w.nV, w.nK], "lu_caching": False},
+                   "grid": [w.nR, w.nP, w.nV, w.nK], "lu_caching": False,
+                   "launch": "hip graph replay" if not args.no_graph else "eager",
+                   "tuning": tuning},
         "roofline": roofline,
         "cpu_baseline": base,
         "phases": phase_out,
+        "eager_ms_per_step": round(eager_ms_per_step, 4),
     }
     if cached is not None:
         out["lu_cached"] = cached

@@ -238,6 +238,35 @@ int verb_get_tuning(verb_ctx* ctx, verb_tuning* t);
 int verb_autotune(verb_ctx* ctx, const verb_grid* g, double* psd, verb_tuning* out);
 int verb_step_graph(verb_ctx* ctx, const verb_grid* g, double* psd);
 
+/* ------------------------------------------------------------------------
+ * Local diffusion with an FP32 band LU and FP64 iterative refinement. No
+ * counterpart in the reference (which factors in FP64 with dgbtrf_); it
+ * solves the same system A x = B0*psd + C0 per (P,R) node for a single
+ * sub-step: the operator is factored in FP32 (partial pivoting, register
+ * window up to kl = nV+1 = 152), then from x = 0 each sweep forms the FP64
+ * residual r = b - A x, stops the node when max|r| <= tol * max|b|, solves
+ * L U d = P r in FP32 and adds d to x in FP64. stats (nullable)
+ * synchronises the stream.
+ * ------------------------------------------------------------------------ */
+typedef struct {
+    double tol;      /* FP64 residual tolerance, relative to max|b| */
+    int max_sweeps;  /* FP32 solves per node at most */
+} verb_mixed_params;
+
+typedef struct {
+    int nodes;           /* systems solved (nodes skipped by include_boundary excluded) */
+    int converged;       /* met tol within max_sweeps */
+    int max_sweeps_used; /* FP32 solves of the slowest node */
+    double mean_sweeps;  /* FP32 solves per node */
+    double max_residual; /* largest final max|r| / max|b| */
+} verb_mixed_stats;
+
+int verb_local_diffusion_mixed(verb_ctx* ctx, const verb_grid* g, double* psd, const verb_mixed_params* params,
+                               verb_mixed_stats* stats);
+/* convection + radial + verb_local_diffusion_mixed */
+int verb_step_mixed(verb_ctx* ctx, const verb_grid* g, double* psd, const verb_mixed_params* params,
+                    verb_mixed_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -425,6 +425,88 @@ int verb_local_diffusion_gmres(verb_ctx* c, const verb_grid* g, double* psd, con
     return VERB_OK;
 }
 
+constexpr int kSlotMixed = 80;  // scratch slots 80..90 for the mixed-precision workspace
+
+int verb_local_diffusion_mixed(verb_ctx* c, const verb_grid* g, double* psd, const verb_mixed_params* prm,
+                               verb_mixed_stats* stats) {
+    if (!c || !grid_ok(g) || !psd || !prm) return VERB_ERR_ARG;
+    if (!(g->sub_dt > 0) || (int)(g->dt / g->sub_dt) > 1)
+        return set_err(c, VERB_ERR_ARG, "mixed-precision path: one sub-step only");
+    if (prm->max_sweeps < 1 || prm->max_sweeps > 64 || !(prm->tol > 0))
+        return set_err(c, VERB_ERR_ARG, "mixed-precision parameters out of range");
+    if (g->nV + 1 > 152) return set_err(c, VERB_ERR_ARG, "mixed-precision path: nV <= 151");
+    const size_t n = (size_t)g->nV * g->nK;
+    const int nodes = g->nP * g->nR;
+    const size_t ldab = 3 * (size_t)(g->nV + 1) + 1;
+    std::lock_guard<std::mutex> lk(c->mu);
+    // the FP64 operator, B0, C0 and b live in the shared local workspace; the
+    // FP64 band is not needed here, so its slot is not requested
+    verb::LocalWork w{};
+    w.b0 = (double*)scratch(c, kSlotLocal + 2, n * nodes * sizeof(double));
+    w.c0 = (double*)scratch(c, kSlotLocal + 3, n * nodes * sizeof(double));
+    w.rhs = (double*)scratch(c, kSlotLocal + 4, n * nodes * sizeof(double));
+    w.diag = (double*)scratch(c, kSlotLocal + 5, 9 * n * nodes * sizeof(double));
+    verb::MixedWork mw;
+    mw.d32 = (float*)scratch(c, kSlotMixed + 0, 9 * n * nodes * sizeof(float));
+    mw.ab32 = (float*)scratch(c, kSlotMixed + 1, ldab * n * nodes * sizeof(float));
+    mw.ipiv = (int64_t*)scratch(c, kSlotMixed + 2, n * nodes * sizeof(int64_t));
+    mw.x = (double*)scratch(c, kSlotMixed + 3, n * nodes * sizeof(double));
+    mw.r32 = (float*)scratch(c, kSlotMixed + 4, n * nodes * sizeof(float));
+    mw.y32 = (float*)scratch(c, kSlotMixed + 5, n * nodes * sizeof(float));
+    mw.c32 = (float*)scratch(c, kSlotMixed + 6, n * nodes * sizeof(float));
+    mw.bmax = (double*)scratch(c, kSlotMixed + 7, nodes * sizeof(double));
+    mw.rres = (double*)scratch(c, kSlotMixed + 8, nodes * sizeof(double));
+    mw.done = (int*)scratch(c, kSlotMixed + 9, nodes * sizeof(int));
+    mw.sweeps = (int*)scratch(c, kSlotMixed + 10, nodes * sizeof(int));
+    if (!w.b0 || !w.c0 || !w.rhs || !w.diag || !mw.d32 || !mw.ab32 || !mw.ipiv || !mw.x || !mw.r32 || !mw.y32 ||
+        !mw.c32 || !mw.bmax || !mw.rres || !mw.done || !mw.sweeps)
+        return set_err(c, VERB_ERR_NOMEM, "device allocation failed");
+    hipEvent_t ev[6];
+    hipEvent_t* marks = nullptr;
+    if (c->timing) {
+        for (auto& e : ev) e = new_event();
+        marks = ev;
+    }
+    int rc = verb::launch_local_mixed(*g, psd, w, mw, prm->tol, prm->max_sweeps, c->status, c->stream, marks);
+    if (marks) {
+        c->marks.push_back({VERB_PHASE_LOCAL_ASSEMBLE, ev[0], ev[1]});
+        c->marks.push_back({VERB_PHASE_LOCAL_FACTOR, ev[2], ev[3]});
+        c->marks.push_back({VERB_PHASE_LOCAL_SOLVE, ev[4], ev[5]});
+    }
+    if (rc) return set_err(c, rc, "mixed-precision local diffusion launch rejected");
+    if (stats) {
+        std::vector<int> done(nodes), sw(nodes);
+        std::vector<double> rr(nodes);
+        HIPCHK(c, hipMemcpyAsync(done.data(), mw.done, nodes * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(sw.data(), mw.sweeps, nodes * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(rr.data(), mw.rres, nodes * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        verb_mixed_stats st{0, 0, 0, 0.0, 0.0};
+        long total = 0;
+        for (int i = 0; i < nodes; i++) {
+            const int iR = i % g->nR;
+            if (!g->include_boundary && (iR == 0 || iR == g->nR - 1)) continue;
+            st.nodes++;
+            if (done[i]) st.converged++;
+            total += sw[i];
+            if (sw[i] > st.max_sweeps_used) st.max_sweeps_used = sw[i];
+            if (rr[i] > st.max_residual) st.max_residual = rr[i];
+        }
+        st.mean_sweeps = st.nodes ? (double)total / st.nodes : 0.0;
+        *stats = st;
+    }
+    return VERB_OK;
+}
+
+int verb_step_mixed(verb_ctx* c, const verb_grid* g, double* psd, const verb_mixed_params* prm,
+                    verb_mixed_stats* stats) {
+    int rc = VERB_OK;
+    if (3 < g->nP || 3 < g->nR) rc = verb_convection_batched(c, g, psd);
+    if (!rc && g->nR >= 3) rc = verb_radial_diffusion_batched(c, g, psd);
+    if (!rc && g->nV >= 3 && g->nK >= 3) rc = verb_local_diffusion_mixed(c, g, psd, prm, stats);
+    return rc;
+}
+
 int verb_step_gmres(verb_ctx* c, const verb_grid* g, double* psd, const verb_gmres_params* prm,
                     verb_gmres_stats* stats) {
     int rc = VERB_OK;

@@ -37,8 +37,6 @@
 
 namespace verb {
 
-typedef double v4d __attribute__((ext_vector_type(4)));
-typedef double v16d __attribute__((ext_vector_type(16)));
 
 int local_lu_variant = 0;
 
@@ -52,65 +50,37 @@ constexpr int NB = 16;
 // (which would wait for every load in turn) and no 64-bit address math.
 constexpr unsigned kOutside = 0x7ffffff8u;  // past the end of any operator buffer
 
-__device__ inline __amdgpu_buffer_rsrc_t operator_rsrc(const double* dg, int n) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(dg), (short)0, 9 * n * (int)sizeof(double),
-                                             0x00020000);
+template <typename T>
+__device__ inline __amdgpu_buffer_rsrc_t operator_rsrc(const T* dg, int n) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(dg), (short)0, 9 * n * (int)sizeof(T), 0x00020000);
 }
 
-__device__ inline double load_entry(__amdgpu_buffer_rsrc_t rs, unsigned off) {
+template <typename T>
+__device__ inline T load_entry(__amdgpu_buffer_rsrc_t rs, unsigned off);
+
+template <>
+__device__ inline double load_entry<double>(__amdgpu_buffer_rsrc_t rs, unsigned off) {
     typedef unsigned u2 __attribute__((ext_vector_type(2)));
     const u2 v = __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0);
     return __hiloint2double((int)v.y, (int)v.x);
 }
 
+template <>
+__device__ inline float load_entry<float>(__amdgpu_buffer_rsrc_t rs, unsigned off) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0));
+}
+
+__device__ inline double load_entry(__amdgpu_buffer_rsrc_t rs, unsigned off) { return load_entry<double>(rs, off); }
+
 // Byte offset of entry (r, c) in the diagonal storage, or kOutside.
+template <typename T>
 __device__ inline unsigned a_off(int n, int nx, int r, int c) {
     const int off = c - r;
     const int dy = (off >= nx - 1) ? 1 : ((off <= 1 - nx) ? -1 : 0);
     const int dx = off - dy * nx;
     const bool ok = (r < n) & (c < n) & (dx >= -1) & (dx <= 1);
-    return ok ? (unsigned)(((dy + 1) * 3 + dx + 1) * n + r) * 8u : kOutside;
+    return ok ? (unsigned)(((dy + 1) * 3 + dx + 1) * n + r) * (unsigned)sizeof(T) : kOutside;
 }
-
-__device__ inline int crow_(int g, int r) { return 4 * r + g; }
-
-// 16x16 tile (row block rb, column block cb) of the operator in accumulator
-// layout (lane jj + 16g, element r: row 4r+g, column jj). A tile that meets
-// exactly one diagonal band (always the case for nx >= 2*NB) has a
-// wave-uniform row offset dy, so each element costs one add, one compare and
-// one select before its load.
-__device__ inline v4d load_tile(__amdgpu_buffer_rsrc_t rs, int n, int nx, int rb, int cb, int jj, int g) {
-    const int d = NB * (cb - rb);
-    const int lo = d - (NB - 1), hi = d + (NB - 1);
-    const bool b0 = (lo <= -nx + 1) & (hi >= -nx - 1);
-    const bool b1 = (lo <= 1) & (hi >= -1);
-    const bool b2 = (lo <= nx + 1) & (hi >= nx - 1);
-    v4d t = {0.0, 0.0, 0.0, 0.0};
-    if ((int)b0 + (int)b1 + (int)b2 == 1) {
-        const int dy = b0 ? -1 : (b1 ? 0 : 1);
-        const bool inside = (NB * rb + NB <= n) & (NB * cb + NB <= n);
-        // dx = d - dy*nx + jj - (4r + g); offset = ((dy+1)*3 + dx + 1)*n + row
-        const int kd = d - dy * nx;
-        const int row0 = NB * rb + g;
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int dx = kd + jj - 4 * r - g;
-            bool ok = (unsigned)(dx + 1) <= 2u;
-            if (!inside) ok = ok & (row0 + 4 * r < n) & (NB * cb + jj < n);
-            const unsigned off = (unsigned)(((dy + 1) * 3 + dx + 1) * n + row0 + 4 * r) * 8u;
-            t[r] = load_entry(rs, ok ? off : kOutside);
-        }
-    } else if (b0 | b1 | b2) {
-#pragma unroll
-        for (int r = 0; r < 4; r++) t[r] = load_entry(rs, a_off(n, nx, NB * rb + crow_(g, r), NB * cb + jj));
-    }
-    return t;
-}
-
-// Accumulator layout of v_mfma_f64_16x16x4f64: lane j + 16*g, element r
-// holds row 4*r + g, column j of the 16x16 tile (rows interleave across the
-// four 16-lane groups).
-__device__ inline int crow(int g, int r) { return 4 * r + g; }
 
 __device__ inline double readlane_d(double v, int lane) {
     const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
@@ -118,7 +88,7 @@ __device__ inline double readlane_d(double v, int lane) {
     return __hiloint2double(hi, lo);
 }
 
-// DPP move of a double within 16-lane rows (ctrl: a row-local DPP pattern).
+// DPP move within 16-lane rows (ctrl: a row-local DPP pattern).
 template <int CTRL>
 __device__ inline double dpp_row(double v) {
     const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
@@ -126,17 +96,92 @@ __device__ inline double dpp_row(double v) {
     return __hiloint2double(hi, lo);
 }
 
+template <int CTRL>
+__device__ inline float dpp_row(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
 // Wavefront maximum, returned as a wave-uniform value: quad_perm [1,0,3,2],
 // quad_perm [2,3,0,1], row_half_mirror and row_mirror leave every lane of a
 // 16-lane row with the row maximum; the four row maxima are read with
 // readlane and combined.
-__device__ inline double wave_max(double v) {
+template <typename T, class RL>
+__device__ inline T wave_max_t(T v, RL readl) {
     v = fmax(v, dpp_row<0xB1>(v));
     v = fmax(v, dpp_row<0x4E>(v));
     v = fmax(v, dpp_row<0x141>(v));
     v = fmax(v, dpp_row<0x140>(v));
-    const double r0 = readlane_d(v, 0), r1 = readlane_d(v, 16), r2 = readlane_d(v, 32), r3 = readlane_d(v, 48);
+    const T r0 = readl(v, 0), r1 = readl(v, 16), r2 = readl(v, 32), r3 = readl(v, 48);
     return fmax(fmax(r0, r1), fmax(r2, r3));
+}
+
+// Arithmetic of the register window in precision T. The accumulator layout of
+// the 16x16x4 MFMA (lane j + 16*g, element r, column j) puts row 4*r + g there
+// for f64 (rows interleave across the four 16-lane groups) and row 4*g + r
+// for f32.
+template <typename T>
+struct Prec;
+
+template <>
+struct Prec<double> {
+    typedef double acc4 __attribute__((ext_vector_type(4)));
+    typedef double vec16 __attribute__((ext_vector_type(16)));
+    static constexpr double maxfinite = 1.7976931348623157e308;
+    __device__ static int crow(int g, int r) { return 4 * r + g; }
+    __device__ static double readlane(double v, int lane) { return readlane_d(v, lane); }
+    __device__ static double wmax(double v) { return wave_max_t(v, [](double x, int q) { return readlane_d(x, q); }); }
+    __device__ static acc4 mfma(double a, double b, acc4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+
+template <>
+struct Prec<float> {
+    typedef float acc4 __attribute__((ext_vector_type(4)));
+    typedef float vec16 __attribute__((ext_vector_type(16)));
+    static constexpr float maxfinite = 3.40282346638528859812e+38f;
+    __device__ static int crow(int g, int r) { return 4 * g + r; }
+    __device__ static float readlane(float v, int lane) {
+        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+    }
+    __device__ static float wmax(float v) { return wave_max_t(v, [](float x, int q) { return readlane(x, q); }); }
+    __device__ static acc4 mfma(float a, float b, acc4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+};
+
+// 16x16 tile (row block rb, column block cb) of the operator in accumulator
+// layout. A tile that meets exactly one diagonal band (always the case for
+// nx >= 2*NB) has a wave-uniform row offset dy, so each element costs one
+// add, one compare and one select before its load.
+template <typename T>
+__device__ inline typename Prec<T>::acc4 load_tile(__amdgpu_buffer_rsrc_t rs, int n, int nx, int rb, int cb, int jj,
+                                                   int g) {
+    using P = Prec<T>;
+    const int d = NB * (cb - rb);
+    const int lo = d - (NB - 1), hi = d + (NB - 1);
+    const bool b0 = (lo <= -nx + 1) & (hi >= -nx - 1);
+    const bool b1 = (lo <= 1) & (hi >= -1);
+    const bool b2 = (lo <= nx + 1) & (hi >= nx - 1);
+    typename P::acc4 t = {T(0), T(0), T(0), T(0)};
+    if ((int)b0 + (int)b1 + (int)b2 == 1) {
+        const int dy = b0 ? -1 : (b1 ? 0 : 1);
+        const bool inside = (NB * rb + NB <= n) & (NB * cb + NB <= n);
+        // dx = d - dy*nx + jj - row; offset = ((dy+1)*3 + dx + 1)*n + row
+        const int kd = d - dy * nx;
+        const int row0 = NB * rb;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int rr = P::crow(g, r);
+            const int dx = kd + jj - rr;
+            bool ok = (unsigned)(dx + 1) <= 2u;
+            if (!inside) ok = ok & (row0 + rr < n) & (NB * cb + jj < n);
+            const unsigned off = (unsigned)(((dy + 1) * 3 + dx + 1) * n + row0 + rr) * (unsigned)sizeof(T);
+            t[r] = load_entry<T>(rs, ok ? off : kOutside);
+        }
+    } else if (b0 | b1 | b2) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) t[r] = load_entry<T>(rs, a_off<T>(n, nx, NB * rb + P::crow(g, r), NB * cb + jj));
+    }
+    return t;
 }
 
 __device__ inline int wave_min_int(int v) {
@@ -149,16 +194,16 @@ __device__ inline int lanes_below(unsigned long long mask) {
     return __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0));
 }
 
-template <int RB, int CB>
+template <typename T, int RB, int CB>
 struct Lds {
     static constexpr int W = NB * (CB - 1);  // window columns right of the panel
     static constexpr int PR = NB * RB;       // window rows
-    double pnl[NB * PR];                     // panel, column-major (full-row interchanges)
-    double pnl2[NB * PR];                    // factorisation scratch, then the dgbtf2-order multipliers
-    double u12[NB * W];
-    double sb[2 * NB * W];                   // pre-interchange contents of the rows that move
-    double stash[CB * 4 * 64];               // wavefront 0's accumulators while it factors
-    double rw[PR];                           // right-hand-side window (index mod PR)
+    T pnl[NB * PR];                          // panel, column-major (full-row interchanges)
+    T pnl2[NB * PR];                         // copy for the dgbtf2-order multipliers
+    T u12[NB * W];
+    T sb[2 * NB * W];                        // pre-interchange contents of the rows that move
+    T stash[CB * 4 * 64];                    // wavefront 0's accumulators while it factors
+    T rw[PR];                                // right-hand-side window (index mod PR)
     int slot[PR];                            // sb row holding window row i's old contents, -1: none
     int dslot[PR];                           // sb row whose contents window row i takes, -1: unchanged
     int jp[NB];
@@ -171,12 +216,15 @@ struct Lds {
 
 __global__ void zero_counter(int* c) { *c = 0; }
 
-template <int RB, int CB>
-__global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgs a) {
-    using L = Lds<RB, CB>;
+template <typename T, int RB, int CB>
+__global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
+    using L = Lds<T, RB, CB>;
+    using P = Prec<T>;
+    using A4 = typename P::acc4;
+    using V16 = typename P::vec16;
     constexpr int W = L::W;
     constexpr int PR = L::PR;
-    extern __shared__ double lds_raw[];
+    extern __shared__ double lds_raw[];  // (8-byte aligned)
     L& S = *reinterpret_cast<L*>(lds_raw);
 
     const int tid = threadIdx.x;
@@ -199,10 +247,10 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgs a) {
     const int stride = a.n;
     const int n = a.nn ? a.nn[node] : a.n, nx = a.nx;
     const int kl = nx + 1, ku = nx + 1, kv = kl + ku, ldab = 2 * kl + ku + 1;
-    const double* dg = a.diag + (size_t)node * 9 * stride;
-    const double* rhs = a.rhs ? a.rhs + (size_t)node * stride : nullptr;
-    double* yv = a.y ? a.y + (size_t)node * stride : nullptr;
-    double* ab = a.ab + (size_t)node * ldab * stride;
+    const T* dg = a.diag + (size_t)node * 9 * stride;
+    const T* rhs = a.rhs ? a.rhs + (size_t)node * stride : nullptr;
+    T* yv = a.y ? a.y + (size_t)node * stride : nullptr;
+    T* ab = a.ab + (size_t)node * ldab * stride;
     int64_t* ipiv = a.ipiv + (size_t)node * stride;
     int* uext = a.uext ? a.uext + (size_t)node * stride : nullptr;
 
@@ -211,12 +259,12 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgs a) {
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int l = tid & 63, g = l >> 4, jj = l & 15;
 
-    const __amdgpu_buffer_rsrc_t rs = operator_rsrc(dg, n);
-    v4d acc[CB];
+    const __amdgpu_buffer_rsrc_t rs = operator_rsrc<T>(dg, n);
+    A4 acc[CB];
     // initial window: row block w, column blocks 0..CB-1 in slots 0..CB-1
 #pragma unroll
-    for (int s = 0; s < CB; s++) acc[s] = load_tile(rs, n, nx, w, s, jj, g);
-    for (int i = tid; i < PR; i += RB * 64) S.rw[i] = (rhs && i < n) ? rhs[i] : 0.0;
+    for (int s = 0; s < CB; s++) acc[s] = load_tile<T>(rs, n, nx, w, s, jj, g);
+    for (int i = tid; i < PR; i += RB * 64) S.rw[i] = (rhs && i < n) ? rhs[i] : T(0);
     if (tid == 0) {
         S.ju = 0;
         S.info = 0;
@@ -235,18 +283,19 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgs a) {
         for (int s = 0; s < CB; s++)
             if (s == s0) {
 #pragma unroll
-                for (int r = 0; r < 4; r++) S.pnl[jj * PR + NB * lb + crow(g, r)] = acc[s][r];
+                for (int r = 0; r < 4; r++) S.pnl[jj * PR + NB * lb + P::crow(g, r)] = acc[s][r];
             }
         __syncthreads();
 
         // 2. panel factorisation by wavefront 0, in registers. Lane l keeps
-        //    window rows l and l+64 as 16-wide vectors; an interchange only
-        //    swaps the rows' logical positions, the pivot row is broadcast
-        //    with readlane, the rank-1 update runs over the full row with the
-        //    pivot-row entries left of the step zeroed, and at the end every
-        //    row is stored at its final position (the result of full-row
-        //    interchanges).
+        //    window rows l + 64q (q < NQ) as 16-wide vectors; an interchange
+        //    only swaps the rows' logical positions, the pivot row is
+        //    broadcast with readlane, the rank-1 update runs over the full row
+        //    with the pivot-row entries left of the step zeroed, and at the
+        //    end every row is stored at its final position (the result of
+        //    full-row interchanges).
         if (w == 0) {
+            constexpr int NQ = (PR + 63) / 64;
 #pragma unroll
             for (int s = 0; s < CB; s++)
 #pragma unroll
@@ -254,94 +303,107 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgs a) {
             int ju = S.ju;
             int info = S.info;
             constexpr int kNone = 1 << 20;
-            const int q0 = l, q1 = l + 64;
-            const bool h0 = q0 < PR, h1 = q1 < PR;
-            v16d x0, x1;
+            V16 x[NQ];
+            int pos[NQ];
+            T rh[NQ];
 #pragma unroll
-            for (int k = 0; k < NB; k++) {
-                x0[k] = h0 ? S.pnl[k * PR + q0] : 0.0;
-                x1[k] = h1 ? S.pnl[k * PR + q1] : 0.0;
+            for (int q = 0; q < NQ; q++) {
+                const int row = l + 64 * q;
+                const bool h = row < PR;
+#pragma unroll
+                for (int k = 0; k < NB; k++) x[q][k] = h ? S.pnl[k * PR + row] : T(0);
+                pos[q] = h ? row : kNone;
+                rh[q] = h ? S.rw[(j0 + row) % PR] : T(0);
             }
-            int pos0 = h0 ? q0 : kNone, pos1 = h1 ? q1 : kNone;
-            double rh0 = h0 ? S.rw[(j0 + q0) % PR] : 0.0;
-            double rh1 = h1 ? S.rw[(j0 + q1) % PR] : 0.0;
             for (int c = 0; c < nc; c++) {
                 const int j = j0 + c;
                 const int km = min(kl, n - 1 - j);
-                const double xc0 = x0[c], xc1 = x1[c];
-                // candidates: logical rows c..c+km
-                const bool c0 = (pos0 >= c) & (pos0 <= c + km);
-                const bool c1 = (pos1 >= c) & (pos1 <= c + km);
-                const double v0 = c0 ? fabs(xc0) : -1.0;
-                const double v1 = c1 ? fabs(xc1) : -1.0;
-                const double m = wave_max(fmax(v0, v1));
-                const bool t0 = c0 & (v0 == m), t1 = c1 & (v1 == m);
-                const unsigned long long tb0 = __ballot(t0), tb1 = __ballot(t1);
-                int jpos;
-                if (__popcll(tb0) + __popcll(tb1) == 1) {
-                    // a unique largest entry (the common case)
-                    jpos = tb0 ? __builtin_amdgcn_readlane(pos0, __ffsll(tb0) - 1)
-                               : __builtin_amdgcn_readlane(pos1, __ffsll(tb1) - 1);
-                } else if (tb0 | tb1) {
-                    // ties: LAPACK takes the first logical row
-                    jpos = wave_min_int(min(t0 ? pos0 : kNone, t1 ? pos1 : kNone));
-                } else {
-                    jpos = c;
+                T xc[NQ], v[NQ];
+                bool cand[NQ];
+                T mloc = T(-1);
+#pragma unroll
+                for (int q = 0; q < NQ; q++) {
+                    xc[q] = x[q][c];
+                    // candidates: logical rows c..c+km
+                    cand[q] = (pos[q] >= c) & (pos[q] <= c + km);
+                    v[q] = cand[q] ? fabs(xc[q]) : T(-1);
+                    mloc = fmax(mloc, v[q]);
                 }
-                const unsigned long long pb0 = __ballot(pos0 == jpos);
-                const unsigned long long pb1 = __ballot(pos1 == jpos);
+                const T m = P::wmax(mloc);
+                unsigned long long tb[NQ];
+                int ntie = 0;
+#pragma unroll
+                for (int q = 0; q < NQ; q++) {
+                    tb[q] = __ballot(cand[q] & (v[q] == m));
+                    ntie += __popcll(tb[q]);
+                }
+                int jpos = c;
+                if (ntie == 1) {
+                    // a unique largest entry (the common case)
+#pragma unroll
+                    for (int q = 0; q < NQ; q++)
+                        if (tb[q]) jpos = __builtin_amdgcn_readlane(pos[q], __ffsll(tb[q]) - 1);
+                } else if (ntie > 1) {
+                    // ties: LAPACK takes the first logical row
+                    int mn = kNone;
+#pragma unroll
+                    for (int q = 0; q < NQ; q++) mn = min(mn, (cand[q] & (v[q] == m)) ? pos[q] : kNone);
+                    jpos = wave_min_int(mn);
+                }
+                // the lane and register set holding logical row jpos
+                int qp = 0, pl = 0;
+#pragma unroll
+                for (int q = NQ - 1; q >= 0; q--) {
+                    const unsigned long long pb = __ballot(pos[q] == jpos);
+                    if (pb) {
+                        qp = q;
+                        pl = __ffsll(pb) - 1;
+                    }
+                }
                 const int jp = jpos - c;
                 if (l == 0) ipiv[j] = (int64_t)(j + jp + 1);
-                const bool in1 = (pb0 == 0);
-                const int pl = in1 ? (__ffsll(pb1) - 1) : (__ffsll(pb0) - 1);
-                const double piv = readlane_d(in1 ? xc1 : xc0, pl);
-                if (piv != 0.0) {
+                T piv = T(0);
+#pragma unroll
+                for (int q = 0; q < NQ; q++)
+                    if (q == qp) piv = P::readlane(xc[q], pl);
+                if (piv != T(0)) {
                     ju = max(ju, min(j + ku + jp, n - 1));
                     // pivot row right of the step (zero at and left of c)
-                    double pr[NB];
-                    double prh;
-                    if (in1) {
+                    T pr[NB];
+                    T prh = T(0);
 #pragma unroll
-                        for (int k = 0; k < NB; k++) pr[k] = readlane_d(x1[k], pl);
-                        prh = readlane_d(rh1, pl);
-                    } else {
+                    for (int q = 0; q < NQ; q++) {
+                        if (q == qp) {
 #pragma unroll
-                        for (int k = 0; k < NB; k++) pr[k] = readlane_d(x0[k], pl);
-                        prh = readlane_d(rh0, pl);
-                    }
-#pragma unroll
-                    for (int k = 0; k < NB; k++) pr[k] = (k > c) ? pr[k] : 0.0;
-                    // interchange: pivot row -> position c, the row at c -> jpos
-                    pos0 = (pos0 == jpos) ? c : ((pos0 == c) ? jpos : pos0);
-                    pos1 = (pos1 == jpos) ? c : ((pos1 == c) ? jpos : pos1);
-                    const double rinv = 1.0 / piv;
-                    const bool e0 = (pos0 > c) & (pos0 <= c + km);
-                    const bool e1 = (pos1 > c) & (pos1 <= c + km);
-                    // multipliers; 0 for rows the step does not touch
-                    const double L0 = e0 ? xc0 * rinv : 0.0;
-                    const double L1 = e1 ? xc1 * rinv : 0.0;
-                    x0[c] = e0 ? L0 : xc0;
-                    x1[c] = e1 ? L1 : xc1;
-                    const double tb = -prh;
-                    if (tb != 0.0) {
-                        rh0 = e0 ? fma(L0, tb, rh0) : rh0;
-                        rh1 = e1 ? fma(L1, tb, rh1) : rh1;
-                    }
-                    if (fabs(rinv) <= 1.7976931348623157e308) {
-                        // finite multipliers: fma(0, t, x) == x for untouched
-                        // rows and fma(L, 0, x) == x for a zero pivot-row entry
-#pragma unroll
-                        for (int k = 0; k < NB; k++) {
-                            x0[k] = fma(L0, -pr[k], x0[k]);
-                            x1[k] = fma(L1, -pr[k], x1[k]);
+                            for (int k = 0; k < NB; k++) pr[k] = P::readlane(x[q][k], pl);
+                            prh = P::readlane(rh[q], pl);
                         }
-                    } else {
+                    }
 #pragma unroll
-                        for (int k = 0; k < NB; k++) {
-                            const double temp = -pr[k];
-                            if (temp != 0.0) {
-                                x0[k] = e0 ? fma(L0, temp, x0[k]) : x0[k];
-                                x1[k] = e1 ? fma(L1, temp, x1[k]) : x1[k];
+                    for (int k = 0; k < NB; k++) pr[k] = (k > c) ? pr[k] : T(0);
+                    const T rinv = T(1) / piv;
+                    const T tb2 = -prh;
+                    const bool finite = fabs(rinv) <= P::maxfinite;
+#pragma unroll
+                    for (int q = 0; q < NQ; q++) {
+                        // interchange: pivot row -> position c, the row at c -> jpos
+                        pos[q] = (pos[q] == jpos) ? c : ((pos[q] == c) ? jpos : pos[q]);
+                        const bool e = (pos[q] > c) & (pos[q] <= c + km);
+                        // multipliers; 0 for rows the step does not touch
+                        const T Lq = e ? xc[q] * rinv : T(0);
+                        x[q][c] = e ? Lq : xc[q];
+                        if (tb2 != T(0)) rh[q] = e ? fma(Lq, tb2, rh[q]) : rh[q];
+                        if (finite) {
+                            // finite multipliers: fma(0, t, x) == x for
+                            // untouched rows and fma(L, 0, x) == x for a zero
+                            // pivot-row entry
+#pragma unroll
+                            for (int k = 0; k < NB; k++) x[q][k] = fma(Lq, -pr[k], x[q][k]);
+                        } else {
+#pragma unroll
+                            for (int k = 0; k < NB; k++) {
+                                const T temp = -pr[k];
+                                if (temp != T(0)) x[q][k] = e ? fma(Lq, temp, x[q][k]) : x[q][k];
                             }
                         }
                     }
@@ -355,32 +417,21 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgs a) {
                     if (uext) uext[j] = ju;
                 }
             }
-            // rows to their final positions
-            if (h0) {
+            // rows to their final positions, and the sb maps: panel rows keep
+            // their index; a row below the panel that moved was pulled up as a
+            // pivot (at most one per column) and takes sb row 16 + its final
+            // position. dslot gives each window row the sb row it takes.
 #pragma unroll
-                for (int k = 0; k < NB; k++) S.pnl[k * PR + pos0] = x0[k];
-                S.rw[(j0 + pos0) % PR] = rh0;
-            }
-            if (h1) {
+            for (int q = 0; q < NQ; q++) {
+                const int row = l + 64 * q;
+                if (row < PR) {
 #pragma unroll
-                for (int k = 0; k < NB; k++) S.pnl[k * PR + pos1] = x1[k];
-                S.rw[(j0 + pos1) % PR] = rh1;
-            }
-            // sb rows: panel rows keep their index; rows below the panel that
-            // moved (pivot rows pulled up, at most one per column) are numbered
-            // after them. dslot gives each window row the sb row it takes.
-            const bool mv0 = h0 && q0 >= NB && pos0 != q0;
-            const bool mv1 = h1 && pos1 != q1;
-            const unsigned long long mb0 = __ballot(mv0), mb1 = __ballot(mv1);
-            const int sl0 = (q0 < NB) ? q0 : (mv0 ? NB + lanes_below(mb0) : -1);
-            const int sl1 = mv1 ? NB + __popcll(mb0) + lanes_below(mb1) : -1;
-            if (h0) {
-                S.slot[q0] = sl0;
-                S.dslot[pos0] = (pos0 < NB || pos0 != q0) ? sl0 : -1;
-            }
-            if (h1) {
-                S.slot[q1] = sl1;
-                S.dslot[pos1] = (pos1 < NB || pos1 != q1) ? sl1 : -1;
+                    for (int k = 0; k < NB; k++) S.pnl[k * PR + pos[q]] = x[q][k];
+                    S.rw[(j0 + pos[q]) % PR] = rh[q];
+                    const int sl = (row < NB) ? row : ((pos[q] != row) ? NB + pos[q] : -1);
+                    S.slot[row] = sl;
+                    S.dslot[pos[q]] = (pos[q] < NB || pos[q] != row) ? sl : -1;
+                }
             }
             if (l < NB && l >= nc) S.jp[l] = 0;
             if (l == 0) {
@@ -400,7 +451,7 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgs a) {
         {
             int sl[4];
 #pragma unroll
-            for (int r = 0; r < 4; r++) sl[r] = S.slot[NB * lb + crow(g, r)];
+            for (int r = 0; r < 4; r++) sl[r] = S.slot[NB * lb + P::crow(g, r)];
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 if (sl[r] >= 0) {
@@ -430,11 +481,11 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgs a) {
 #pragma unroll
             for (int s = 0; s < CB; s++) {
                 const int cb = (p + 1) + ((s - (p + 1) % CB) + CB) % CB;
-                acc[s] = load_tile(rs, n, nx, rb, cb, jj, g);
+                acc[s] = load_tile<T>(rs, n, nx, rb, cb, jj, g);
             }
             for (int c = l; c < NB; c += 64) {
                 const int i = NB * rb + c;
-                S.rw[i % PR] = (rhs && i < n) ? rhs[i] : 0.0;
+                S.rw[i % PR] = (rhs && i < n) ? rhs[i] : T(0);
             }
         }
 
@@ -442,17 +493,17 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgs a) {
         //     do not hold the panel rows
         if (lb != 0) {
             for (int t = NB * 4 * (lb - 1) + l; t < W; t += 64 * (RB - 1)) {
-                double u[NB];
+                T u[NB];
 #pragma unroll
-                for (int c = 0; c < NB; c++) u[c] = (c < nc) ? S.sb[S.dslot[c] * W + t] : 0.0;
+                for (int c = 0; c < NB; c++) u[c] = (c < nc) ? S.sb[S.dslot[c] * W + t] : T(0);
 #pragma unroll
                 for (int c = 1; c < NB; c++) {
                     // one row of L11 at a time keeps the register footprint small
                     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                     for (int m = 0; m < c; m++) {
-                        const double temp = -u[m];
-                        if (temp != 0.0) u[c] += S.pnl[m * PR + c] * temp;
+                        const T temp = -u[m];
+                        if (temp != T(0)) u[c] += S.pnl[m * PR + c] * temp;
                     }
                 }
 #pragma unroll
@@ -463,7 +514,7 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgs a) {
         if (lb != 0) {
             int ds[4];
 #pragma unroll
-            for (int r = 0; r < 4; r++) ds[r] = S.dslot[NB * lb + crow(g, r)];
+            for (int r = 0; r < 4; r++) ds[r] = S.dslot[NB * lb + P::crow(g, r)];
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 if (ds[r] >= 0) {
@@ -481,7 +532,7 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgs a) {
         // 4. trailing update; the finished U rows stream out
         if (lb != 0) {
             const int rowbase = NB * lb;
-            double an[4];
+            T an[4];
 #pragma unroll
             for (int kk = 0; kk < 4; kk++) an[kk] = -S.pnl[(4 * kk + g) * PR + rowbase + jj];
 #pragma unroll
@@ -489,16 +540,16 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgs a) {
                 if (s == s0) continue;
                 const int lc = (s - s0 + CB) % CB;
                 if (NB * (p + lc) > ju) continue;
-                v4d c4 = acc[s];
+                A4 c4 = acc[s];
 #pragma unroll
                 for (int kk = 0; kk < 4; kk++) {
-                    const double bv = S.u12[(4 * kk + g) * W + NB * (lc - 1) + jj];
-                    c4 = __builtin_amdgcn_mfma_f64_16x16x4f64(an[kk], bv, c4, 0, 0, 0);
+                    const T bv = S.u12[(4 * kk + g) * W + NB * (lc - 1) + jj];
+                    c4 = P::mfma(an[kk], bv, c4);
                 }
                 acc[s] = c4;
             }
             // column block p+CB enters the window in slot s0
-            const v4d nt = load_tile(rs, n, nx, p + lb, p + CB, jj, g);
+            const A4 nt = load_tile<T>(rs, n, nx, p + lb, p + CB, jj, g);
 #pragma unroll
             for (int s = 0; s < CB; s++)
                 if (s == s0) acc[s] = nt;
@@ -514,7 +565,7 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgs a) {
                     const int col_off = e >> 4;
                     const int col = j0 + col_off;
                     if (col >= r && col <= ue) {
-                        const double v = (col_off < NB) ? S.pnl[col_off * PR + c] : S.u12[c * W + (col_off - NB)];
+                        const T v = (col_off < NB) ? S.pnl[col_off * PR + c] : S.u12[c * W + (col_off - NB)];
                         ab[(size_t)col * ldab + kv + r - col] = v;
                     }
                 }
@@ -525,7 +576,7 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgs a) {
             for (int c = nc - 1; c >= 0; c--) {
                 const int jp = S.jp[c];
                 if (jp != 0 && l < c) {
-                    const double t = S.pnl2[l * PR + c];
+                    const T t = S.pnl2[l * PR + c];
                     S.pnl2[l * PR + c] = S.pnl2[l * PR + c + jp];
                     S.pnl2[l * PR + c + jp] = t;
                 }
@@ -536,7 +587,7 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgs a) {
             for (int c = 0; c < nc; c++) {
                 const int km = min(kl, n - 1 - (j0 + c));
                 for (int i = l; i < kl; i += 64)
-                    ab[(size_t)(j0 + c) * ldab + kv + 1 + i] = (i < km) ? S.pnl2[c * PR + c + 1 + i] : 0.0;
+                    ab[(size_t)(j0 + c) * ldab + kv + 1 + i] = (i < km) ? S.pnl2[c * PR + c + 1 + i] : T(0);
             }
         }
         __syncthreads();
@@ -558,12 +609,13 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgs a) {
 // first, broadcasting each x with readlane.
 // ---------------------------------------------------------------------------
 constexpr int kBsThreads = 256;
-constexpr int kBsMaxI = 15;  // staged 16-column groups per thread: (kl+ku+16)/16 <= 15
-constexpr int kXRing = 256;  // x entries kept in LDS: >= kl+ku+16, a power of two
+constexpr int kBsMaxI = 20;  // staged 16-column groups per thread: (kl+ku+16)/16 <= 20
+constexpr int kXRing = 512;  // x entries kept in LDS: >= kl+ku+32, a power of two
 
-__global__ __launch_bounds__(kBsThreads) void band_backsolve(const double* __restrict__ ab_all,
-                                                             const double* __restrict__ y_all,
-                                                             double* __restrict__ x_all, int n, int kl, int ku,
+template <typename T>
+__global__ __launch_bounds__(kBsThreads) void band_backsolve(const T* __restrict__ ab_all,
+                                                             const T* __restrict__ y_all,
+                                                             T* __restrict__ x_all, int n, int kl, int ku,
                                                              int ldab, int nR, int include_boundary,
                                                              const int* __restrict__ nn,
                                                              const int* __restrict__ order,
@@ -576,38 +628,39 @@ __global__ __launch_bounds__(kBsThreads) void band_backsolve(const double* __res
     const int kv = kl + ku;
     const int WS = kv + NB;  // staged columns per block (relative to the block start)
     const int nI = (WS + NB - 1) / NB;
-    const double* ab = ab_all + (size_t)node * ldab * stride;
-    const double* yv = y_all + (size_t)node * stride;
-    double* xo = x_all + (size_t)node * stride;
-    extern __shared__ double sh[];
-    double* x = sh;                    // ring of kXRing: x[i] at i % kXRing
-    double* stg = sh + kXRing;         // 2 x [NB][WS]
-    double* part = stg + 2 * NB * WS;  // [NB][16] partial sums
+    const T* ab = ab_all + (size_t)node * ldab * stride;
+    const T* yv = y_all + (size_t)node * stride;
+    T* xo = x_all + (size_t)node * stride;
+    extern __shared__ double sh_raw[];  // 8-byte aligned
+    T* sh = reinterpret_cast<T*>(sh_raw);
+    T* x = sh;                    // ring of kXRing: x[i] at i % kXRing
+    T* stg = sh + kXRing;         // 2 x [NB][WS]
+    T* part = stg + 2 * NB * WS;  // [NB][16] partial sums
     const int tid = threadIdx.x;
     const int ri = tid & 15, t = tid >> 4;
     const __amdgpu_buffer_rsrc_t rs =
-        __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(ab), (short)0, ldab * n * (int)sizeof(double), 0x00020000);
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(ab), (short)0, ldab * n * (int)sizeof(T), 0x00020000);
     const int nblk = (n + NB - 1) / NB;
 
-    double va[kBsMaxI], vb[kBsMaxI];
-    double ya = 0.0, yb = 0.0;
+    T va[kBsMaxI], vb[kBsMaxI];
+    T ya = T(0), yb = T(0);
     // this thread's entries of block blk: row jb+ri, columns jb + t + 16i
     // (and, for the 16 threads of the triangle, y of its row)
-    auto issue = [&](int blk, double (&v)[kBsMaxI], double& yr) {
+    auto issue = [&](int blk, T (&v)[kBsMaxI], T& yr) {
         const int jb = NB * blk, r = jb + ri;
-        if (tid < NB) yr = (r < n) ? yv[r] : 0.0;
+        if (tid < NB) yr = (r < n) ? yv[r] : T(0);
         const int hi = min(r + kv, n - 1);
 #pragma unroll
         for (int i = 0; i < kBsMaxI; i++) {
             const int co = t + NB * i;
             const int col = jb + co;
             const bool ok = (i < nI) & (co < WS) & (r < n) & (col >= r) & (col <= hi);
-            const unsigned off = (unsigned)(col * (ldab - 1) + kv + r) * 8u;
-            v[i] = load_entry(rs, ok ? off : kOutside);
+            const unsigned off = (unsigned)(col * (ldab - 1) + kv + r) * (unsigned)sizeof(T);
+            v[i] = load_entry<T>(rs, ok ? off : kOutside);
         }
     };
-    auto commit = [&](int buf, const double (&v)[kBsMaxI]) {
-        double* st = stg + buf * NB * WS;
+    auto commit = [&](int buf, const T (&v)[kBsMaxI]) {
+        T* st = stg + buf * NB * WS;
 #pragma unroll
         for (int i = 0; i < kBsMaxI; i++) {
             const int co = t + NB * i;
@@ -615,11 +668,11 @@ __global__ __launch_bounds__(kBsThreads) void band_backsolve(const double* __res
         }
     };
     // one block: partial sums over the band right of it, then the triangle
-    auto solve_block = [&](int blk, double yr) {
+    auto solve_block = [&](int blk, T yr) {
         const int jb = NB * blk;
-        const double* st = stg + (blk & 1) * NB * WS;
+        const T* st = stg + (blk & 1) * NB * WS;
         {
-            double sacc = 0.0;
+            T sacc = T(0);
             const int r = jb + ri;
             if (r < n) {
                 const int hi = min(r + kv, n - 1) - jb;
@@ -632,23 +685,23 @@ __global__ __launch_bounds__(kBsThreads) void band_backsolve(const double* __res
             const int i = tid & 15;
             const int r = jb + i;
             const bool live = (tid < NB) & (r < n);
-            double b = 0.0, rd = 0.0;
-            double urow[NB];
+            T b = T(0), rd = T(0);
+            T urow[NB];
 #pragma unroll
-            for (int k = 0; k < NB; k++) urow[k] = 0.0;
+            for (int k = 0; k < NB; k++) urow[k] = T(0);
             if (live) {
-                double sp = 0.0;
+                T sp = T(0);
 #pragma unroll
                 for (int q = 0; q < 16; q++) sp += part[i * 16 + q];
                 b = yr - sp;
 #pragma unroll
                 for (int k = 0; k < NB; k++) urow[k] = st[i * WS + k];
-                rd = 1.0 / st[i * WS + i];
+                rd = T(1) / st[i * WS + i];
             }
 #pragma unroll
             for (int k = NB - 1; k >= 0; k--) {
                 if (jb + k < n) {
-                    const double xk = readlane_d(b * rd, k);
+                    const T xk = Prec<T>::readlane(b * rd, k);
                     if (tid == k) {
                         x[(jb + k) & (kXRing - 1)] = xk;
                         xo[jb + k] = xk;
@@ -663,7 +716,7 @@ __global__ __launch_bounds__(kBsThreads) void band_backsolve(const double* __res
     // is solved from LDS, block blk-1 waits in registers and block blk-2 is
     // in flight
     const int top = nblk - 1;
-    double ycur = 0.0;
+    T ycur = T(0);
     issue(top, va, ycur);
     commit(top & 1, va);
     if (top >= 1) issue(top - 1, vb, yb);
@@ -699,56 +752,61 @@ __global__ __launch_bounds__(kBsThreads) void band_backsolve(const double* __res
 // ---------------------------------------------------------------------------
 constexpr int kFwdBlock = 16;
 
-__global__ __launch_bounds__(64, 1) void band_forward(const double* __restrict__ ab_all,
+template <typename T, int NW>
+__global__ __launch_bounds__(64, 1) void band_forward(const T* __restrict__ ab_all,
                                                    const int64_t* __restrict__ ipiv_all, int n, int kl, int ku,
                                                    int ldab, const double* __restrict__ b0_all,
                                                    const double* __restrict__ c0_all,
-                                                   const double* __restrict__ rhs_all, double* __restrict__ y_all,
-                                                   int nR, int include_boundary) {
+                                                   const T* __restrict__ rhs_all, T* __restrict__ y_all,
+                                                   int nR, int include_boundary, const int* __restrict__ done) {
     const int node = blockIdx.x;
     if (nR > 0 && !include_boundary && (node % nR == 0 || node % nR == nR - 1)) return;
+    if (done && done[node]) return;
     const int kd = kl + ku;
-    const double* ab = ab_all + (size_t)node * ldab * n;
+    const T* ab = ab_all + (size_t)node * ldab * n;
     const int64_t* ipiv = ipiv_all + (size_t)node * n;
-    const double* rhs = rhs_all + (size_t)node * n;
+    const T* rhs = rhs_all + (size_t)node * n;
     const double* B0 = b0_all ? b0_all + (size_t)node * n : nullptr;
     const double* C0 = c0_all ? c0_all + (size_t)node * n : nullptr;
-    double* y = y_all + (size_t)node * n;
+    T* y = y_all + (size_t)node * n;
     const int l = threadIdx.x;
     const __amdgpu_buffer_rsrc_t rs =
-        __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(ab), (short)0, ldab * n * (int)sizeof(double), 0x00020000);
-    auto b_at = [&](int i) -> double {
-        if (i >= n) return 0.0;
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(ab), (short)0, ldab * n * (int)sizeof(T), 0x00020000);
+    auto b_at = [&](int i) -> T {
+        if (i >= n) return T(0);
         if (!B0) return rhs[i];
-        return __dadd_rn(__dmul_rn(rhs[i], B0[i]), C0[i]);
+        return (T)__dadd_rn(__dmul_rn((double)rhs[i], B0[i]), C0[i]);
     };
-    double w0 = b_at(l), w1 = b_at(64 + l);
+    // window positions 64q + l, q < NW, of the entries starting at column j0
+    T w[NW];
+#pragma unroll
+    for (int q = 0; q < NW; q++) w[q] = b_at(64 * q + l);
     const int ncol = n - 1;  // columns with an elimination step
     const int nblk = (ncol + kFwdBlock - 1) / kFwdBlock;
-    // multipliers of block blk for this lane: column j0+c, the rows that
-    // window positions l and 64+l are at (i = position - c - 1)
-    double La0[kFwdBlock], Lb0[kFwdBlock], La1[kFwdBlock], Lb1[kFwdBlock], La2[kFwdBlock], Lb2[kFwdBlock];
+    T L0[NW][kFwdBlock], L1[NW][kFwdBlock], L2[NW][kFwdBlock];
     int pv0 = 0, pv1 = 0, pv2 = 0;
-    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
-    // multipliers of block blk for this lane: column j0+c, the rows that
-    // window positions l and 64+l are at (i = position - c - 1)
-    auto fetch = [&](int blk, double (&La)[kFwdBlock], double (&Lb)[kFwdBlock], int& pv, double& fr) {
+    T f0 = T(0), f1 = T(0), f2 = T(0);
+    // multipliers of block blk for this lane: column j0+c at the rows of
+    // window positions 64q + l (i = position - c - 1), the pivots of the
+    // block, and the entries that enter the window after the block
+    auto fetch = [&](int blk, T (&La)[NW][kFwdBlock], int& pv, T& fr) {
         const int j0 = kFwdBlock * blk;
-        fr = (l >= 64 - kFwdBlock) ? b_at(j0 + kFwdBlock + 64 + l) : 0.0;  // enters the window after this block
+        fr = (l >= 64 - kFwdBlock) ? b_at(j0 + kFwdBlock + 64 * (NW - 1) + l) : T(0);
 #pragma unroll
         for (int c = 0; c < kFwdBlock; c++) {
             const int j = j0 + c;
             const int lm = min(kl, n - 1 - j);
-            const int i0 = l - c - 1, i1 = 63 + l - c;
-            const bool ok0 = (j < ncol) & (i0 >= 0) & (i0 < lm);
-            const bool ok1 = (j < ncol) & (i1 >= 0) & (i1 < lm);
             const unsigned base = (unsigned)(j * ldab + kd + 1);
-            La[c] = load_entry(rs, ok0 ? (base + (unsigned)i0) * 8u : kOutside);
-            Lb[c] = load_entry(rs, ok1 ? (base + (unsigned)i1) * 8u : kOutside);
+#pragma unroll
+            for (int q = 0; q < NW; q++) {
+                const int i = 64 * q + l - c - 1;
+                const bool ok = (j < ncol) & (i >= 0) & (i < lm);
+                La[q][c] = load_entry<T>(rs, ok ? (base + (unsigned)i) * (unsigned)sizeof(T) : kOutside);
+            }
         }
         pv = (l < kFwdBlock && j0 + l < ncol) ? (int)ipiv[j0 + l] - 1 - j0 : l;
     };
-    auto process = [&](int blk, const double (&La)[kFwdBlock], const double (&Lb)[kFwdBlock], int pv, double fresh) {
+    auto process = [&](int blk, const T (&La)[NW][kFwdBlock], int pv, T fresh) {
         const int j0 = kFwdBlock * blk;
 #pragma unroll
         for (int c = 0; c < kFwdBlock; c++) {
@@ -757,55 +815,97 @@ __global__ __launch_bounds__(64, 1) void band_forward(const double* __restrict__
                 const int lm = min(kl, n - 1 - j);
                 const int p = __builtin_amdgcn_readlane(pv, c);  // window position of row ipiv(j)
                 if (p != c) {
-                    const double vc = readlane_d(w0, c);
-                    const double vp = (p < 64) ? readlane_d(w0, p) : readlane_d(w1, p - 64);
-                    if (l == c) w0 = vp;
-                    if (p < 64) {
-                        if (l == p) w0 = vc;
-                    } else {
-                        if (l == p - 64) w1 = vc;
-                    }
+                    const T vc = Prec<T>::readlane(w[0], c);
+                    T vp = T(0);
+#pragma unroll
+                    for (int q = 0; q < NW; q++)
+                        if ((p >> 6) == q) vp = Prec<T>::readlane(w[q], p & 63);
+                    if (l == c) w[0] = vp;
+#pragma unroll
+                    for (int q = 0; q < NW; q++)
+                        if ((p >> 6) == q && l == (p & 63)) w[q] = vc;
                 }
-                const double bj = readlane_d(w0, c);
-                if (bj != 0.0) {
-                    const double t = -bj;
+                const T bj = Prec<T>::readlane(w[0], c);
+                if (bj != T(0)) {
+                    const T t = -bj;
                     // only rows j+1..j+lm change
-                    w0 = (l > c && l <= c + lm) ? fma(La[c], t, w0) : w0;
-                    w1 = (63 + l - c < lm) ? fma(Lb[c], t, w1) : w1;
+#pragma unroll
+                    for (int q = 0; q < NW; q++) {
+                        const int i = 64 * q + l - c - 1;
+                        w[q] = (i >= 0 && i < lm) ? fma(La[q][c], t, w[q]) : w[q];
+                    }
                 }
             }
         }
         // columns j0..j0+15 are final: out, and the window moves by 16
-        if (l < kFwdBlock && j0 + l < n) y[j0 + l] = w0;
-        const double s0 = __shfl(w0, (l + kFwdBlock) & 63);
-        const double s1 = __shfl(w1, (l + kFwdBlock) & 63);
-        w0 = (l < 64 - kFwdBlock) ? s0 : s1;
-        w1 = (l < 64 - kFwdBlock) ? s1 : fresh;
+        if (l < kFwdBlock && j0 + l < n) y[j0 + l] = w[0];
+        T sh[NW];
+#pragma unroll
+        for (int q = 0; q < NW; q++) sh[q] = __shfl(w[q], (l + kFwdBlock) & 63);
+#pragma unroll
+        for (int q = 0; q < NW; q++) w[q] = (l < 64 - kFwdBlock) ? sh[q] : (q + 1 < NW ? sh[q + 1 < NW ? q + 1 : q] : fresh);
     };
     // multipliers are requested two blocks ahead of their use
-    if (nblk > 0) fetch(0, La0, Lb0, pv0, f0);
-    if (nblk > 1) fetch(1, La1, Lb1, pv1, f1);
+    if (nblk > 0) fetch(0, L0, pv0, f0);
+    if (nblk > 1) fetch(1, L1, pv1, f1);
     for (int blk = 0; blk < nblk; blk += 3) {
-        if (blk + 2 < nblk) fetch(blk + 2, La2, Lb2, pv2, f2);
-        process(blk, La0, Lb0, pv0, f0);
+        if (blk + 2 < nblk) fetch(blk + 2, L2, pv2, f2);
+        process(blk, L0, pv0, f0);
         if (blk + 1 >= nblk) break;
-        if (blk + 3 < nblk) fetch(blk + 3, La0, Lb0, pv0, f0);
-        process(blk + 1, La1, Lb1, pv1, f1);
+        if (blk + 3 < nblk) fetch(blk + 3, L0, pv0, f0);
+        process(blk + 1, L1, pv1, f1);
         if (blk + 2 >= nblk) break;
-        if (blk + 4 < nblk) fetch(blk + 4, La1, Lb1, pv1, f1);
-        process(blk + 2, La2, Lb2, pv2, f2);
+        if (blk + 4 < nblk) fetch(blk + 4, L1, pv1, f1);
+        process(blk + 2, L2, pv2, f2);
     }
     const int j0 = kFwdBlock * nblk;
-    if (j0 + l < n) y[j0 + l] = w0;
-    if (j0 + 64 + l < n) y[j0 + 64 + l] = w1;
+#pragma unroll
+    for (int q = 0; q < NW; q++)
+        if (j0 + 64 * q + l < n) y[j0 + 64 * q + l] = w[q];
+}
+
+template <typename T>
+int launch_forward_t(const T* ab, const int64_t* ipiv, int n, int kl, int ku, int ldab, const double* b0,
+                     const double* c0, const T* rhs, T* y, int nodes, int nR, int include_boundary,
+                     hipStream_t stream, const int* done = nullptr) {
+    if ((size_t)ldab * n * sizeof(T) >= 0x7ffffff8u) return VERB_ERR_ARG;
+    if (kl + kFwdBlock <= 128) {
+        hipLaunchKernelGGL((band_forward<T, 2>), dim3(nodes), dim3(64), 0, stream, ab, ipiv, n, kl, ku, ldab, b0, c0,
+                           rhs, y, nR, include_boundary, done);
+    } else if (kl + kFwdBlock <= 192) {
+        hipLaunchKernelGGL((band_forward<T, 3>), dim3(nodes), dim3(64), 0, stream, ab, ipiv, n, kl, ku, ldab, b0, c0,
+                           rhs, y, nR, include_boundary, done);
+    } else {
+        return VERB_ERR_ARG;
+    }
+    return hipGetLastError() == hipSuccess ? VERB_OK : VERB_ERR_HIP;
 }
 
 int launch_forward(const double* ab, const int64_t* ipiv, int n, int kl, int ku, int ldab, const double* b0,
                    const double* c0, const double* rhs, double* y, int nodes, int nR, int include_boundary,
                    hipStream_t stream) {
-    if (kl + kFwdBlock > 128 || (size_t)ldab * n * sizeof(double) >= 0x7ffffff8u) return VERB_ERR_ARG;
-    hipLaunchKernelGGL(band_forward, dim3(nodes), dim3(64), 0, stream, ab, ipiv, n, kl, ku, ldab, b0, c0, rhs, y, nR,
-                       include_boundary);
+    return launch_forward_t<double>(ab, ipiv, n, kl, ku, ldab, b0, c0, rhs, y, nodes, nR, include_boundary, stream);
+}
+
+int launch_forward32(const float* ab, const int64_t* ipiv, int n, int kl, int ku, int ldab, const float* rhs,
+                     float* y, int nodes, int nR, int include_boundary, hipStream_t stream, const int* done) {
+    return launch_forward_t<float>(ab, ipiv, n, kl, ku, ldab, nullptr, nullptr, rhs, y, nodes, nR, include_boundary,
+                                   stream, done);
+}
+
+template <typename T>
+int launch_backsolve_t(const T* ab, const T* y, T* x, int n, int kl, int ku, int ldab, int nodes, int nR,
+                       int include_boundary, hipStream_t stream, const int* nn, const int* order, const int* done) {
+    if (kl + ku + NB > NB * kBsMaxI || kl + ku + 2 * NB > kXRing) return VERB_ERR_ARG;
+    const int WS = kl + ku + NB;
+    const size_t lds = sizeof(T) * ((size_t)kXRing + 2 * NB * WS + NB * 16);
+    int dev = 0, max_lds = 0;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    if (lds + 256 > (size_t)max_lds) return VERB_ERR_ARG;
+    hipFuncSetAttribute((const void*)band_backsolve<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(band_backsolve<T>, dim3(nodes), dim3(kBsThreads), lds, stream, ab, y, x, n, kl, ku, ldab, nR,
+                       include_boundary, nn, order, done);
     return hipGetLastError() == hipSuccess ? VERB_OK : VERB_ERR_HIP;
 }
 
@@ -813,55 +913,63 @@ int launch_backsolve(const double* ab, const double* y, const int* uext, double*
                      int nodes, int nR, int include_boundary, hipStream_t stream, const int* nn, const int* order,
                      const int* done) {
     (void)uext;  // U rows are stored up to kl+ku past the diagonal, so extents are not needed
-    if (kl + ku + NB > NB * kBsMaxI || kl + ku + 2 * NB > kXRing) return VERB_ERR_ARG;
-    const int WS = kl + ku + NB;
-    const size_t lds = sizeof(double) * ((size_t)kXRing + 2 * NB * WS + NB * 16);
+    return launch_backsolve_t<double>(ab, y, x, n, kl, ku, ldab, nodes, nR, include_boundary, stream, nn, order,
+                                      done);
+}
+
+int launch_backsolve32(const float* ab, const float* y, float* x, int n, int kl, int ku, int ldab, int nodes, int nR,
+                       int include_boundary, hipStream_t stream, const int* done) {
+    return launch_backsolve_t<float>(ab, y, x, n, kl, ku, ldab, nodes, nR, include_boundary, stream, nullptr, nullptr,
+                                     done);
+}
+
+template <typename T, int RB, int CB>
+int launch_band_lu_t(const LuArgsT<T>& a, int nodes, hipStream_t stream) {
+    const size_t lds = sizeof(Lds<T, RB, CB>);
     int dev = 0, max_lds = 0;
     hipGetDevice(&dev);
     hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-    if (lds + 256 > (size_t)max_lds) return VERB_ERR_ARG;
-    hipFuncSetAttribute((const void*)band_backsolve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(band_backsolve, dim3(nodes), dim3(kBsThreads), lds, stream, ab, y, x, n, kl, ku, ldab, nR,
-                       include_boundary, nn, order, done);
-    return hipGetLastError() == hipSuccess ? VERB_OK : VERB_ERR_HIP;
-}
-
-template <int RB, int CB>
-size_t band_lu_lds_bytes() {
-    return sizeof(Lds<RB, CB>);
-}
-
-template <int RB, int CB>
-int launch_band_lu_t(const LuArgs& a, int nodes, hipStream_t stream) {
-    const size_t lds = band_lu_lds_bytes<RB, CB>();
-    hipFuncSetAttribute((const void*)band_lu_window<RB, CB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (lds > (size_t)max_lds) return VERB_ERR_ARG;
+    hipFuncSetAttribute((const void*)band_lu_window<T, RB, CB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     int grid = nodes;
     if (a.queue) {
-        int dev = 0, cus = 0;
-        hipGetDevice(&dev);
+        int cus = 0;
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         grid = min(nodes, max(1, cus));
         hipLaunchKernelGGL(zero_counter, dim3(1), dim3(1), 0, stream, a.queue);
     }
-    hipLaunchKernelGGL((band_lu_window<RB, CB>), dim3(grid), dim3(RB * 64), lds, stream, a);
+    hipLaunchKernelGGL((band_lu_window<T, RB, CB>), dim3(grid), dim3(RB * 64), lds, stream, a);
     return hipGetLastError() == hipSuccess ? VERB_OK : VERB_ERR_HIP;
 }
 
 namespace {
 
+template <typename T>
 struct LuVariant {
     int rb, cb;
-    int (*launch)(const LuArgs&, int, hipStream_t);
+    int (*launch)(const LuArgsT<T>&, int, hipStream_t);
 };
 
-// Register-window instantiations: RB*16 >= kl+16 rows, CB*16 >= 2kl+16 columns.
-const LuVariant kLuVariants[] = {
-    {3, 5, launch_band_lu_t<3, 5>},   {4, 7, launch_band_lu_t<4, 7>},   {5, 9, launch_band_lu_t<5, 9>},
-    {6, 11, launch_band_lu_t<6, 11>}, {7, 13, launch_band_lu_t<7, 13>}, {8, 14, launch_band_lu_t<8, 14>},
-    {8, 15, launch_band_lu_t<8, 15>},
+// Register-window instantiations: RB*16 >= kl+16 rows, CB*16 >= 2kl+16
+// columns. FP64 keeps CB*8 accumulator registers per lane and stops at
+// kl = 112; FP32 needs half as many and reaches kl = 152 with 11 wavefronts.
+const LuVariant<double> kLuVariants[] = {
+    {3, 5, launch_band_lu_t<double, 3, 5>},   {4, 7, launch_band_lu_t<double, 4, 7>},
+    {5, 9, launch_band_lu_t<double, 5, 9>},   {6, 11, launch_band_lu_t<double, 6, 11>},
+    {7, 13, launch_band_lu_t<double, 7, 13>}, {8, 14, launch_band_lu_t<double, 8, 14>},
+    {8, 15, launch_band_lu_t<double, 8, 15>},
+};
+const LuVariant<float> kLuVariants32[] = {
+    {5, 9, launch_band_lu_t<float, 5, 9>},
+    {7, 13, launch_band_lu_t<float, 7, 13>},  {8, 14, launch_band_lu_t<float, 8, 14>},
+    {8, 15, launch_band_lu_t<float, 8, 15>},  {9, 17, launch_band_lu_t<float, 9, 17>},
+    {10, 19, launch_band_lu_t<float, 10, 19>}, {11, 20, launch_band_lu_t<float, 11, 20>},
 };
 
-bool lu_variant_fits(const LuVariant& v, int kl) { return NB * v.rb >= kl + NB && NB * v.cb >= 2 * kl + NB; }
+template <typename T>
+bool lu_variant_fits(const LuVariant<T>& v, int kl) {
+    return NB * v.rb >= kl + NB && NB * v.cb >= 2 * kl + NB;
+}
 
 }  // namespace
 
@@ -882,7 +990,14 @@ int launch_band_lu(const LuArgs& a, int nodes, hipStream_t stream, int variant) 
         if (variant >= lu_variant_count() || !lu_variant_fits(kLuVariants[variant], kl)) return VERB_ERR_ARG;
         return kLuVariants[variant].launch(a, nodes, stream);
     }
-    for (const LuVariant& v : kLuVariants)
+    for (const auto& v : kLuVariants)
+        if (lu_variant_fits(v, kl)) return v.launch(a, nodes, stream);
+    return VERB_ERR_ARG;
+}
+
+int launch_band_lu32(const LuArgsT<float>& a, int nodes, hipStream_t stream) {
+    const int kl = a.nx + 1;
+    for (const auto& v : kLuVariants32)
         if (lu_variant_fits(v, kl)) return v.launch(a, nodes, stream);
     return VERB_ERR_ARG;
 }

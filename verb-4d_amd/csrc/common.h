@@ -40,7 +40,11 @@ struct LocalWork {
     int lu_variant = -1;   // register-window instantiation (-1: smallest that covers kl)
 };
 
-struct LuArgs;
+template <typename T>
+struct LuArgsT;
+using LuArgs = LuArgsT<double>;
+// FP32 factorisation (register window up to kl = 152)
+int launch_band_lu32(const LuArgsT<float>& a, int nodes, hipStream_t stream);
 int launch_band_lu(const LuArgs& a, int nodes, hipStream_t stream, int variant = -1);
 // register-window instantiations: count, and (rb, cb) of entry i; returns
 // whether entry i covers kl
@@ -54,6 +58,12 @@ int launch_backsolve(const double* ab, const double* y, const int* uext, double*
 int launch_forward(const double* ab, const int64_t* ipiv, int n, int kl, int ku, int ldab, const double* b0,
                    const double* c0, const double* rhs, double* y, int nodes, int nR, int include_boundary,
                    hipStream_t stream);
+// FP32 solves with FP32 band factors (no B0/C0 premultiplication)
+int launch_forward32(const float* ab, const int64_t* ipiv, int n, int kl, int ku, int ldab, const float* rhs,
+                     float* y, int nodes, int nR, int include_boundary, hipStream_t stream,
+                     const int* done = nullptr);
+int launch_backsolve32(const float* ab, const float* y, float* x, int n, int kl, int ku, int ldab, int nodes, int nR,
+                       int include_boundary, hipStream_t stream, const int* done = nullptr);
 // 0 = automatic (register-window LU where an instantiation covers kl),
 // 1 = force the one-column-per-step reference-order kernel.
 extern int local_lu_variant;
@@ -86,6 +96,25 @@ struct GmresWork {
 int launch_local_gmres(const verb_grid& g, double* psd, const LocalWork& w, const GmresWork& gw, double tol,
                        int restart, int max_cycles, double inner_tol, int* status, hipStream_t stream,
                        hipEvent_t* marks = nullptr);
+
+// Local diffusion with an FP32 band LU and FP64 iterative refinement:
+// x0 = 0; repeat { r = b - A x (FP64); stop the node when max|r| <= tol*max|b|;
+// solve L U d = P r in FP32; x += d }. Single sub-step only.
+struct MixedWork {
+    float* d32;      // [nodes][9][n] FP32 operator
+    float* ab32;     // [nodes][ldab][n] FP32 factors (L in dgbtf2 order)
+    int64_t* ipiv;   // [nodes][n]
+    double* x;       // [nodes][n] FP64 iterate (energy-fast)
+    float* r32;      // [nodes][n] residual
+    float* y32;      // [nodes][n] forward-eliminated residual
+    float* c32;      // [nodes][n] correction
+    double* bmax;    // [nodes] max|b|
+    double* rres;    // [nodes] last max|r| / max|b|
+    int* done;       // [nodes] 1 once converged (or skipped)
+    int* sweeps;     // [nodes] solves used
+};
+int launch_local_mixed(const verb_grid& g, double* psd, const LocalWork& w, const MixedWork& mw, double tol,
+                       int max_sweeps, int* status, hipStream_t stream, hipEvent_t* marks = nullptr);
 
 // Single-system LAPACK band routines on device buffers (one system).
 int launch_dgbtrf(int n, int kl, int ku, double* ab, int ldab, int64_t* ipiv, int* status, hipStream_t stream);

@@ -398,6 +398,95 @@ __global__ void gather_psd(const verb_grid g, const double* __restrict__ psd, do
     x[(size_t)node * nfull + iy * nx + ix] = psd[(size_t)node * nfull + (size_t)ix * g.nK + iy];
 }
 
+// ---- mixed precision (FP32 factors, FP64 refinement) ----
+
+constexpr int kMxThreads = 256;
+
+// d32 = (float)diag, x = 0, done = skipped node, sweeps = 0, bmax = max|b|
+__global__ __launch_bounds__(kMxThreads) void mixed_init(const verb_grid g, const double* __restrict__ diag,
+                                                         const double* __restrict__ b, float* __restrict__ d32,
+                                                         double* __restrict__ x, double* __restrict__ bmax,
+                                                         int* __restrict__ done, int* __restrict__ sweeps) {
+    const int node = blockIdx.x;
+    const int iR = node % g.nR;
+    const int n = g.nV * g.nK;
+    const size_t base = (size_t)node * n;
+    if (node_skipped(g, iR)) {
+        if (threadIdx.x == 0) {
+            done[node] = 1;
+            sweeps[node] = 0;
+            bmax[node] = 0.0;
+        }
+        return;
+    }
+    for (int i = threadIdx.x; i < 9 * n; i += kMxThreads) d32[9 * base + i] = (float)diag[9 * base + i];
+    double m = 0.0;
+    for (int i = threadIdx.x; i < n; i += kMxThreads) {
+        x[base + i] = 0.0;
+        m = fmax(m, fabs(b[base + i]));
+    }
+    __shared__ double red[kMxThreads / 64];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int q = 0; q < kMxThreads / 64; q++) t = fmax(t, red[q]);
+        bmax[node] = t;
+        done[node] = 0;
+        sweeps[node] = 0;
+    }
+}
+
+// r = b - A x in FP64 for the nodes still refining; a node whose
+// max|r| <= tol * max|b| is done. count: whether a solve follows.
+__global__ __launch_bounds__(kMxThreads) void mixed_residual(const verb_grid g, const double* __restrict__ diag,
+                                                             const double* __restrict__ b,
+                                                             const double* __restrict__ x, float* __restrict__ r32,
+                                                             const double* __restrict__ bmax, double tol,
+                                                             double* __restrict__ rres, int* __restrict__ done,
+                                                             int* __restrict__ sweeps, int count) {
+    const int node = blockIdx.x;
+    if (done[node]) return;
+    const int nx = g.nV, n = g.nV * g.nK;
+    const size_t base = (size_t)node * n;
+    const double* dg = diag + 9 * base;
+    double m = 0.0;
+    for (int i = threadIdx.x; i < n; i += kMxThreads) {
+        double acc = b[base + i];
+#pragma unroll
+        for (int s = 0; s < 9; s++) {
+            const int j = i + (s / 3 - 1) * nx + (s % 3 - 1);
+            if (j >= 0 && j < n) acc = fma(-dg[(size_t)s * n + i], x[base + j], acc);
+        }
+        r32[base + i] = (float)acc;
+        m = (acc == acc) ? fmax(m, fabs(acc)) : (double)INFINITY;  // a NaN residual never converges
+    }
+    __shared__ double red[kMxThreads / 64];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int q = 0; q < kMxThreads / 64; q++) t = fmax(t, red[q]);
+        const double bm = bmax[node];
+        rres[node] = bm > 0.0 ? t / bm : t;
+        if (t <= tol * bm) done[node] = 1;
+        else if (count) sweeps[node] += 1;
+    }
+}
+
+__global__ void mixed_update(const verb_grid g, double* __restrict__ x, const float* __restrict__ c,
+                             const int* __restrict__ done) {
+    const int node = blockIdx.y;
+    if (done[node]) return;
+    const int n = g.nV * g.nK;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[(size_t)node * n + i] += (double)c[(size_t)node * n + i];
+}
+
 int solve_chunk(int n, int width) {
     int dev = 0, max_lds = 0;
     hipGetDevice(&dev);
@@ -572,6 +661,56 @@ int launch_local_gmres(const verb_grid& g, double* psd, const LocalWork& w, cons
                           nullptr, gw.conv);
     if (rc) return rc;
     hipLaunchKernelGGL(scatter_psd, agrid, dim3(kAsmThreads), 0, stream, g, gw.x, psd, (const int*)nullptr);
+    if (marks) hipEventRecord(marks[5], stream);
+    return hipGetLastError() == hipSuccess ? VERB_OK : VERB_ERR_HIP;
+}
+
+
+int launch_local_mixed(const verb_grid& g, double* psd, const LocalWork& w, const MixedWork& mw, double tol,
+                       int max_sweeps, int* status, hipStream_t stream, hipEvent_t* marks) {
+    if (g.nV < 3 || g.nK < 3 || max_sweeps < 1) return VERB_ERR_ARG;
+    const int nx = g.nV, ny = g.nK, n = nx * ny;
+    const int kl = nx + 1, ku = nx + 1, ldab = 2 * kl + ku + 1;
+    const int nodes = g.nP * g.nR;
+    const int num_substeps = max(1, (int)(g.dt / g.sub_dt));
+    if (num_substeps != 1) return VERB_ERR_ARG;
+    const double dt_sub = g.dt;
+    dim3 agrid((n + kAsmThreads - 1) / kAsmThreads, nodes);
+    if (marks) hipEventRecord(marks[0], stream);
+    hipLaunchKernelGGL(local_assemble, agrid, dim3(kAsmThreads), 0, stream, g, psd, w.ab, w.b0, w.c0, w.rhs, w.diag,
+                       dt_sub, 0, 1, status, (const int*)nullptr);
+    hipLaunchKernelGGL(mixed_init, dim3(nodes), dim3(kMxThreads), 0, stream, g, w.diag, w.rhs, mw.d32, mw.x,
+                       mw.bmax, mw.done, mw.sweeps);
+    if (marks) {
+        hipEventRecord(marks[1], stream);
+        hipEventRecord(marks[2], stream);
+    }
+    LuArgsT<float> la;
+    la.diag = mw.d32; la.rhs = nullptr; la.y = nullptr; la.ab = mw.ab32; la.ipiv = mw.ipiv; la.uext = nullptr;
+    la.n = n; la.nx = nx; la.write_L = 1; la.status = status; la.nR = g.nR;
+    la.include_boundary = g.include_boundary;
+    la.nn = nullptr; la.order = nullptr;
+    int rc = launch_band_lu32(la, nodes, stream);
+    if (rc) return rc;
+    if (marks) {
+        hipEventRecord(marks[3], stream);
+        hipEventRecord(marks[4], stream);
+    }
+    dim3 ugrid((n + 255) / 256, nodes);
+    for (int s = 0; s <= max_sweeps; s++) {
+        const int solve = s < max_sweeps;
+        hipLaunchKernelGGL(mixed_residual, dim3(nodes), dim3(kMxThreads), 0, stream, g, w.diag, w.rhs, mw.x, mw.r32,
+                           mw.bmax, tol, mw.rres, mw.done, mw.sweeps, solve);
+        if (!solve) break;
+        rc = launch_forward32(mw.ab32, mw.ipiv, n, kl, ku, ldab, mw.r32, mw.y32, nodes, g.nR, g.include_boundary,
+                              stream, mw.done);
+        if (rc) return rc;
+        rc = launch_backsolve32(mw.ab32, mw.y32, mw.c32, n, kl, ku, ldab, nodes, g.nR, g.include_boundary, stream,
+                                mw.done);
+        if (rc) return rc;
+        hipLaunchKernelGGL(mixed_update, ugrid, dim3(256), 0, stream, g, mw.x, mw.c32, mw.done);
+    }
+    hipLaunchKernelGGL(scatter_psd, agrid, dim3(kAsmThreads), 0, stream, g, mw.x, psd, (const int*)nullptr);
     if (marks) hipEventRecord(marks[5], stream);
     return hipGetLastError() == hipSuccess ? VERB_OK : VERB_ERR_HIP;
 }

@@ -62,6 +62,22 @@ class GmresStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class MixedParams(ctypes.Structure):
+    """ctypes mirror of ``verb_mixed_params``."""
+
+    _fields_ = (("tol", ctypes.c_double), ("max_sweeps", ctypes.c_int))
+
+
+class MixedStats(ctypes.Structure):
+    """ctypes mirror of ``verb_mixed_stats``."""
+
+    _fields_ = (("nodes", ctypes.c_int), ("converged", ctypes.c_int), ("max_sweeps_used", ctypes.c_int),
+                ("mean_sweeps", ctypes.c_double), ("max_residual", ctypes.c_double))
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class Tuning(ctypes.Structure):
     """ctypes mirror of ``verb_tuning``."""
 
@@ -149,6 +165,10 @@ _SIGNATURES = [
      [_P, ctypes.POINTER(VerbGrid), ctypes.c_void_p, ctypes.POINTER(GmresParams), ctypes.POINTER(GmresStats)]),
     ("verb_step_gmres", ctypes.c_int,
      [_P, ctypes.POINTER(VerbGrid), ctypes.c_void_p, ctypes.POINTER(GmresParams), ctypes.POINTER(GmresStats)]),
+    ("verb_local_diffusion_mixed", ctypes.c_int,
+     [_P, ctypes.POINTER(VerbGrid), ctypes.c_void_p, ctypes.POINTER(MixedParams), ctypes.POINTER(MixedStats)]),
+    ("verb_step_mixed", ctypes.c_int,
+     [_P, ctypes.POINTER(VerbGrid), ctypes.c_void_p, ctypes.POINTER(MixedParams), ctypes.POINTER(MixedStats)]),
     ("verb_set_tuning", ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),
     ("verb_get_tuning", ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),
     ("verb_autotune", ctypes.c_int, [_P, ctypes.POINTER(VerbGrid), ctypes.c_void_p, ctypes.POINTER(Tuning)]),

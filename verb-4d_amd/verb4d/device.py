@@ -8,7 +8,7 @@ import ctypes
 
 import numpy as np
 
-from . import Context, GmresParams, GmresStats, Tuning, VerbGrid, lib
+from . import Context, GmresParams, GmresStats, MixedParams, MixedStats, Tuning, VerbGrid, lib
 
 
 class DeviceGrid:
@@ -117,6 +117,22 @@ class DeviceGrid:
         p = params or self.gmres_params()
         st = GmresStats() if stats else None
         self.ctx.check(lib().verb_step_gmres(self.ctx.handle, ctypes.byref(self.grid), self.psd.data_ptr(),
+                                             ctypes.byref(p), ctypes.byref(st) if st else None))
+        return st.as_dict() if st else None
+
+    def local_mixed(self, tol=1e-12, max_sweeps=8, stats=True):
+        """Local diffusion with FP32 band LU + FP64 iterative refinement;
+        returns the stats dict (synchronises) or None."""
+        p = MixedParams(tol, max_sweeps)
+        st = MixedStats() if stats else None
+        self.ctx.check(lib().verb_local_diffusion_mixed(self.ctx.handle, ctypes.byref(self.grid), self.psd.data_ptr(),
+                                                        ctypes.byref(p), ctypes.byref(st) if st else None))
+        return st.as_dict() if st else None
+
+    def step_mixed(self, tol=1e-12, max_sweeps=8, stats=False):
+        p = MixedParams(tol, max_sweeps)
+        st = MixedStats() if stats else None
+        self.ctx.check(lib().verb_step_mixed(self.ctx.handle, ctypes.byref(self.grid), self.psd.data_ptr(),
                                              ctypes.byref(p), ctypes.byref(st) if st else None))
         return st.as_dict() if st else None
 

@@ -78,7 +78,7 @@ def dll(L, kp):
 class Workload:
     """All arrays of one configuration ([P][R][V][K] float64 unless noted)."""
 
-    def __init__(self, nP, nR, nV, nK, seed, kp=3.0, kp0=None, dt=300.0, sub_dt=None, rich=False):
+    def __init__(self, nP, nR, nV, nK, seed, kp=3.0, kp0=None, dt=300.0, sub_dt=None, rich=False, stiff=False):
         self.nP, self.nR, self.nV, self.nK = nP, nR, nV, nK
         self.seed = seed
         self.kp0 = kp if kp0 is None else kp0
@@ -106,8 +106,18 @@ class Workload:
         self.f["Kg"] = np.ascontiguousarray(Ag)
         self.psd0 = np.ascontiguousarray(np.exp(-((Lg - 4.5) ** 2) / 0.5) * np.exp(-Eg / 0.3) * np.sin(Ag))
         n = nP * nR * nV * nK
-        self.daa = (10.0 ** (-7.0 + 4.0 * splitmix_u01(seed, FIELD_DAA, n))).reshape(shape)
-        self.dee = (10.0 ** (-7.0 + 4.0 * splitmix_u01(seed, FIELD_DEE, n))).reshape(shape)
+        if stiff:
+            # Daa log-uniform in [1e-9, 1] /s, largest within ~5 degrees of
+            # the loss cone (two decades lower far from it); DEE log-uniform
+            # in [1e-8, 1e-2] /s
+            alc = loss_cone(Lg)
+            near = np.exp(-(((Ag - alc) / math.radians(5.0)) ** 2))
+            self.daa = (10.0 ** (-9.0 + 9.0 * splitmix_u01(seed, FIELD_DAA, n))).reshape(shape) * (
+                0.01 + 0.99 * near)
+            self.dee = (10.0 ** (-8.0 + 6.0 * splitmix_u01(seed, FIELD_DEE, n))).reshape(shape)
+        else:
+            self.daa = (10.0 ** (-7.0 + 4.0 * splitmix_u01(seed, FIELD_DAA, n))).reshape(shape)
+            self.dee = (10.0 ** (-7.0 + 4.0 * splitmix_u01(seed, FIELD_DEE, n))).reshape(shape)
         self.dae_u = (2.0 * splitmix_u01(seed, FIELD_DAE, n) - 1.0).reshape(shape)
         p2 = Eg * (Eg + 2.0 * 0.511)
         sa, ca = np.sin(Ag), np.cos(Ag)
@@ -184,6 +194,13 @@ def config0(seed=1, rich=False):
 def config1(seed=1001, rich=False):
     """29x24x101x91, Kp 2 -> 6 at step 50, dt = 300 s, 100 steps."""
     return Workload(nP=24, nR=29, nV=101, nK=91, seed=seed, kp=2.0, kp0=2.0, dt=300.0, rich=rich)
+
+
+def config3(seed=3001, nP=24, nR=29):
+    """Stiff mixed-precision case: 29x24x151x121, Daa log-uniform in
+    1e-9..1 /s peaking near the loss cone, DEE log-uniform in 1e-8..1e-2 /s,
+    dt = 600 s (kl = ku = 152 in the reference's energy-fast ordering)."""
+    return Workload(nP=nP, nR=nR, nV=151, nK=121, seed=seed, kp=2.0, kp0=2.0, dt=600.0, stiff=True)
 
 
 def kp_schedule_config1(step):
