@@ -147,6 +147,8 @@ def main():
     ap.add_argument("--no-gmres", action="store_true",
                     help="skip timing steps whose local solve is the matrix-free GMRES path")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--no-uneven", action="store_true",
+                    help="skip the uneven-extent production grid (loss-cone pitch-angle extents, 4040..9191 unknowns)")
     ap.add_argument("--no-graph", action="store_true",
                     help="time eagerly launched steps instead of the captured HIP graph of the step")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -288,6 +290,34 @@ def main():
               "ms_per_step": round(ev0.elapsed_time(ev1) / args.steps, 4), "tol": 1e-11,
               "first_step": st}
 
+    uneven = None
+    if not args.no_uneven:
+        # same grid, pitch-angle extents following the loss cone plus a drift
+        # band (4040..9191 unknowns per node), Kp 2 / 6 across MLT; persistent
+        # LU queue over nodes sorted by size, replayed from its graph
+        w2, k0 = W.config2(seed=2001)
+        dg2 = DeviceGrid(w2, ctx=ctx)
+        dg2.step_graph(k0)  # eager step + capture
+        for _ in range(max(1, args.warmup)):
+            dg2.step_graph(k0)
+        dg2.check_status()
+        ctx.synchronize()
+        torch.cuda.synchronize()
+        ev0.record(ext)
+        for _ in range(args.steps):
+            dg2.step_graph(k0)
+        ev1.record(ext)
+        ctx.synchronize()
+        torch.cuda.synchronize()
+        dg2.check_status()
+        nn = w2.nV * (w2.nK - k0)
+        uneven = {"steps_per_s": round(1000.0 * args.steps / ev0.elapsed_time(ev1), 4),
+                  "ms_per_step": round(ev0.elapsed_time(ev1) / args.steps, 4),
+                  "unknowns_min": int(nn.min()), "unknowns_max": int(nn.max()),
+                  "unknowns_mean": round(float(nn.mean()), 1), "kp": "2 (MLT 0-11), 6 (MLT 12-23)"}
+        del dg2
+        torch.cuda.empty_cache()
+
     base = None
     if not args.no_cpu_baseline:
         threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
@@ -322,6 +352,8 @@ def main():
         out["lu_cached"] = cached
     if gm is not None:
         out["gmres_local"] = gm
+    if uneven is not None:
+        out["uneven_extents"] = uneven
     if base:
         out["speedup_vs_cpu"] = round(value / base["value"], 2)
     print(json.dumps(out), flush=True)

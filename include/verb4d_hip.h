@@ -222,8 +222,10 @@ int verb_step_gmres(verb_ctx* ctx, const verb_grid* g, double* psd, const verb_g
  *     queue) on this grid and PSD, keeps the fastest in the ctx and reports
  *     them; psd is restored before it returns.
  *   verb_step_graph: the plain step (convection, radial, local with LU caching
- *     off) of verb_step. The first call for a grid/psd/tuning runs it eagerly
- *     and records it as a HIP graph; later calls replay the graph.
+ *     off) of verb_step, or of verb_step_uneven when k0 is given (host array,
+ *     one first active pitch-angle column per node). The first call for a
+ *     grid/psd/tuning/k0 runs it eagerly and records it as a HIP graph; later
+ *     calls replay the graph.
  * ------------------------------------------------------------------------ */
 typedef struct {
     int convection_tq;    /* (V,K) planes per convection workgroup, 0 = automatic */
@@ -236,7 +238,7 @@ typedef struct {
 int verb_set_tuning(verb_ctx* ctx, const verb_tuning* t);
 int verb_get_tuning(verb_ctx* ctx, verb_tuning* t);
 int verb_autotune(verb_ctx* ctx, const verb_grid* g, double* psd, verb_tuning* out);
-int verb_step_graph(verb_ctx* ctx, const verb_grid* g, double* psd);
+int verb_step_graph(verb_ctx* ctx, const verb_grid* g, double* psd, const int* k0);
 
 /* ------------------------------------------------------------------------
  * Local diffusion with an FP32 band LU and FP64 iterative refinement. No

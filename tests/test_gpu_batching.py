@@ -110,3 +110,30 @@ def test_graph_replay_matches_eager_steps(ctx):
         # eager work between replays
         dg.set_psd(got)
     dg.set_tuning()
+
+
+def test_graph_replay_uneven_extents(ctx):
+    """The uneven step (loss-cone extents, persistent LU queue) replayed from
+    its graph equals the eager uneven step, and the extents table is uploaded
+    once."""
+    say("graph replay, uneven extents")
+    w = W.config0(seed=45, rich=True)
+    rng = np.random.default_rng(9)
+    k0 = rng.integers(0, 14, size=(w.nP, w.nR)).astype(np.int32)
+    dg = DeviceGrid(w, ctx=ctx)
+    dg.set_tuning(lu_persistent=1)
+    dg.set_psd(w.psd0)
+    eager = []
+    for _ in range(3):
+        dg.step_uneven(k0)
+        dg.check_status()
+        eager.append(dg.psd_host())
+    dg.set_psd(w.psd0)
+    for i in range(3):
+        dg.step_graph(k0)
+        dg.check_status()
+        got = dg.psd_host()
+        assert bit_equal(got, eager[i]), i
+        dg.set_psd(got)
+    dg.set_tuning()
+

@@ -42,6 +42,9 @@ struct verb_ctx {
     verb_grid graph_grid{};
     double* graph_psd = nullptr;
     int graph_tune[3] = {0, -1, 0};
+    std::vector<int> graph_k0;  // pitch-angle extents the graph was captured for (empty: uniform)
+    // the uneven extents last uploaded (first column, unknowns, dispatch order)
+    std::vector<int> uneven_k0;
 };
 
 namespace {
@@ -332,11 +335,16 @@ int verb_local_diffusion_uneven(verb_ctx* c, const verb_grid* g, double* psd, co
     int* dn = (int*)scratch(c, 51, nodes * sizeof(int));
     int* dord = (int*)scratch(c, 52, nodes * sizeof(int));
     if (!dk || !dn || !dord) return set_err(c, VERB_ERR_NOMEM, "device allocation failed");
-    HIPCHK(c, hipMemcpyAsync(dk, hk.data(), nodes * sizeof(int), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(dn, hn.data(), nodes * sizeof(int), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(dord, ord.data(), nodes * sizeof(int), hipMemcpyHostToDevice, c->stream));
-    // the host vectors die at return: wait for the copies
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->uneven_k0 != hk) {
+        // new extents: upload them once; a call with the same extents launches
+        // kernels only (so the step can be captured into a graph)
+        HIPCHK(c, hipMemcpyAsync(dk, hk.data(), nodes * sizeof(int), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(dn, hn.data(), nodes * sizeof(int), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(dord, ord.data(), nodes * sizeof(int), hipMemcpyHostToDevice, c->stream));
+        // the host vectors die at return: wait for the copies
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        c->uneven_k0 = hk;
+    }
     LocalBuffers lb = local_buffers(c, g, lu_cache, ipiv_cache, kSlotLocal);
     if (lb.rc) return lb.rc;
     const bool factor = (lu_cache == nullptr) || recompute_lu;
@@ -635,11 +643,17 @@ int verb_autotune(verb_ctx* c, const verb_grid* g, double* psd, verb_tuning* out
     return rc;
 }
 
-int verb_step_graph(verb_ctx* c, const verb_grid* g, double* psd) {
+int verb_step_graph(verb_ctx* c, const verb_grid* g, double* psd, const int* k0) {
     if (!c || !grid_ok(g) || !psd) return VERB_ERR_ARG;
     const int tune[3] = {c->conv_tq, c->lu_variant, c->lu_persistent};
+    std::vector<int> key;
+    if (k0) key.assign(k0, k0 + g->nP * g->nR);
     const bool match = c->graph_exec && std::memcmp(&c->graph_grid, g, sizeof(verb_grid)) == 0 &&
-                       c->graph_psd == psd && std::memcmp(c->graph_tune, tune, sizeof(tune)) == 0;
+                       c->graph_psd == psd && std::memcmp(c->graph_tune, tune, sizeof(tune)) == 0 &&
+                       c->graph_k0 == key;
+    auto run = [&] {
+        return k0 ? verb_step_uneven(c, g, psd, k0, 1, 1, 1) : verb_step(c, g, psd, 1, 1, 1, nullptr, nullptr, 1);
+    };
     if (match) {
         HIPCHK(c, hipGraphLaunch(c->graph_exec, c->stream));
         return VERB_OK;
@@ -647,7 +661,7 @@ int verb_step_graph(verb_ctx* c, const verb_grid* g, double* psd) {
     // A new shape: this call runs the step eagerly (which also allocates the
     // workspace outside any capture), then records the same launch sequence
     // into a graph for the following calls.
-    int rc = verb_step(c, g, psd, 1, 1, 1, nullptr, nullptr, 1);
+    int rc = run();
     if (rc) return rc;
     if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
     if (c->graph) hipGraphDestroy(c->graph);
@@ -656,7 +670,7 @@ int verb_step_graph(verb_ctx* c, const verb_grid* g, double* psd) {
     const bool saved_timing = c->timing;
     c->timing = false;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    rc = verb_step(c, g, psd, 1, 1, 1, nullptr, nullptr, 1);
+    rc = run();
     hipGraph_t gr = nullptr;
     const hipError_t e = hipStreamEndCapture(c->stream, &gr);
     c->timing = saved_timing;
@@ -674,6 +688,7 @@ int verb_step_graph(verb_ctx* c, const verb_grid* g, double* psd) {
     c->graph_grid = *g;
     c->graph_psd = psd;
     std::memcpy(c->graph_tune, tune, sizeof(tune));
+    c->graph_k0 = key;
     return VERB_OK;
 }
 

@@ -172,7 +172,7 @@ _SIGNATURES = [
     ("verb_set_tuning", ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),
     ("verb_get_tuning", ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),
     ("verb_autotune", ctypes.c_int, [_P, ctypes.POINTER(VerbGrid), ctypes.c_void_p, ctypes.POINTER(Tuning)]),
-    ("verb_step_graph", ctypes.c_int, [_P, ctypes.POINTER(VerbGrid), ctypes.c_void_p]),
+    ("verb_step_graph", ctypes.c_int, [_P, ctypes.POINTER(VerbGrid), ctypes.c_void_p, ctypes.c_void_p]),
 ]
 
 

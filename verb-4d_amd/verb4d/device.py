@@ -148,10 +148,15 @@ class DeviceGrid:
         t = Tuning(convection_tq, lu_rb, lu_cb, lu_persistent, 0.0, 0.0)
         self.ctx.check(lib().verb_set_tuning(self.ctx.handle, ctypes.byref(t)))
 
-    def step_graph(self):
-        """The plain step through a captured HIP graph (captured on the first
-        call for this grid, replayed afterwards)."""
-        self.ctx.check(lib().verb_step_graph(self.ctx.handle, ctypes.byref(self.grid), self.psd.data_ptr()))
+    def step_graph(self, k0=None):
+        """The plain step (uneven pitch-angle extents k0 [P][R] when given)
+        through a captured HIP graph: captured on the first call for this
+        grid, replayed afterwards."""
+        kp = None
+        if k0 is not None:
+            self._k0_keep = np.ascontiguousarray(k0, dtype=np.int32).ravel()
+            kp = self._k0_keep.ctypes.data_as(ctypes.c_void_p)
+        self.ctx.check(lib().verb_step_graph(self.ctx.handle, ctypes.byref(self.grid), self.psd.data_ptr(), kp))
 
     def check_status(self):
         return self.ctx.check_status()

@@ -78,7 +78,8 @@ def dll(L, kp):
 class Workload:
     """All arrays of one configuration ([P][R][V][K] float64 unless noted)."""
 
-    def __init__(self, nP, nR, nV, nK, seed, kp=3.0, kp0=None, dt=300.0, sub_dt=None, rich=False, stiff=False):
+    def __init__(self, nP, nR, nV, nK, seed, kp=3.0, kp0=None, dt=300.0, sub_dt=None, rich=False, stiff=False,
+                 kp_by_mlt=None):
         self.nP, self.nR, self.nV, self.nK = nP, nR, nV, nK
         self.seed = seed
         self.kp0 = kp if kp0 is None else kp0
@@ -151,18 +152,25 @@ class Workload:
         self.radial_sources = 0.0
         self.local_sources = 1.0
         self.include_boundary = 1
+        # optional Kp per MLT sector (array of nP): mixed regimes across MLT
+        self.kp_by_mlt = None if kp_by_mlt is None else np.asarray(kp_by_mlt, dtype=np.float64)
         self.set_kp(kp)
 
     def set_kp(self, kp):
         """Coefficients that follow Kp: local D's, D_LL and the drift field."""
         self.kp = float(kp)
-        s = 10.0 ** ((self.kp - self.kp0) / 4.0)
+        if self.kp_by_mlt is not None:
+            kpp = self.kp_by_mlt[:, None, None, None]
+            s = 10.0 ** ((kpp - self.kp0) / 4.0)
+        else:
+            kpp = self.kp
+            s = 10.0 ** ((self.kp - self.kp0) / 4.0)
         Eg = self.f["Vg"]
         self.f["DKK"] = np.ascontiguousarray(self.daa * s)
         self.f["DVV"] = np.ascontiguousarray(self.dee * s * Eg ** 2)
         self.f["DVK"] = np.ascontiguousarray(0.5 * np.sqrt(self.daa * s * self.dee * s) * self.dae_u * Eg)
-        self.f["DLL"] = np.ascontiguousarray(dll(self.f["Lg"], self.kp))
-        VP, VR = volland_stern(self.f["Pg"], self.f["Rg"], self.kp)
+        self.f["DLL"] = np.ascontiguousarray(dll(self.f["Lg"], kpp))
+        VP, VR = volland_stern(self.f["Pg"], self.f["Rg"], kpp)
         self.f["VP"] = np.ascontiguousarray(VP)
         self.f["VR"] = np.ascontiguousarray(VR)
 
@@ -194,6 +202,26 @@ def config0(seed=1, rich=False):
 def config1(seed=1001, rich=False):
     """29x24x101x91, Kp 2 -> 6 at step 50, dt = 300 s, 100 steps."""
     return Workload(nP=24, nR=29, nV=101, nK=91, seed=seed, kp=2.0, kp0=2.0, dt=300.0, rich=rich)
+
+
+FIELD_BAND = 9
+
+
+def config2(seed=2001):
+    """Uneven systems on the production grid: returns (workload, k0) with k0
+    [P][R] the first active pitch-angle column of each node. The active
+    count follows the bounce loss cone, 40 cells at R = 1.5 rising linearly
+    to 91 at R = 6.6, plus an MLT-dependent drift-loss band of -10..+10
+    cells drawn from the seed (clipped to 40..91), so systems have
+    101*40 = 4040 to 101*91 = 9191 unknowns. Kp is 2 on the night side
+    (MLT sectors 0..11) and 6 on the day side (12..23)."""
+    nP, nR, nV, nK = 24, 29, 101, 91
+    kp = np.where(np.arange(nP) < nP // 2, 2.0, 6.0)
+    w = Workload(nP=nP, nR=nR, nV=nV, nK=nK, seed=seed, kp=2.0, kp0=2.0, dt=300.0, kp_by_mlt=kp)
+    base = np.round(40.0 + (nK - 40.0) * (w.L1 - 1.5) / (6.6 - 1.5))
+    band = np.floor(21.0 * splitmix_u01(seed, FIELD_BAND, nP * nR)).reshape(nP, nR) - 10.0
+    active = np.clip(base[None, :] + band, 40, nK).astype(np.int32)
+    return w, (nK - active).astype(np.int32)
 
 
 def config3(seed=3001, nP=24, nR=29):
