@@ -190,6 +190,17 @@ __device__ inline int wave_min_int(int v) {
     return __builtin_amdgcn_readfirstlane(v);
 }
 
+// Lane index read by an opaque instruction sequence. The panel loop takes its
+// lane-dependent indices from this at the start of each phase: computed once
+// outside the loop, the many LDS addresses derived from them stay live across
+// all phases and are spilled to scratch, whose reloads wait on every
+// outstanding vector-memory load.
+__device__ inline int lane_fresh() {
+    int v;
+    asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(v));
+    return v;
+}
+
 __device__ inline int lanes_below(unsigned long long mask) {
     return __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0));
 }
@@ -198,8 +209,7 @@ template <typename T, int RB, int CB>
 struct Lds {
     static constexpr int W = NB * (CB - 1);  // window columns right of the panel
     static constexpr int PR = NB * RB;       // window rows
-    T pnl[NB * PR];                          // panel, column-major (full-row interchanges)
-    T pnl2[NB * PR];                         // copy for the dgbtf2-order multipliers
+    T pnl[2][NB * PR];                       // panels p and p+1, column-major (full-row interchanges)
     T u12[NB * W];
     T sb[2 * NB * W];                        // pre-interchange contents of the rows that move
     T stash[CB * 4 * 64];                    // wavefront 0's accumulators while it factors
@@ -207,7 +217,6 @@ struct Lds {
     int slot[PR];                            // sb row holding window row i's old contents, -1: none
     int dslot[PR];                           // sb row whose contents window row i takes, -1: unchanged
     int jp[NB];
-    int uext[NB];                            // U row extents of the panel rows
     int ju, info;
     int qnode;                               // persistent mode: current dispatch slot
 };
@@ -257,7 +266,7 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
     // wavefront index as a uniform value: per-wavefront branches below are
     // scalar, so the register allocator sees them as exclusive
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int l = tid & 63, g = l >> 4, jj = l & 15;
+    const int l = lane_fresh(), g = l >> 4, jj = l & 15;
 
     const __amdgpu_buffer_rsrc_t rs = operator_rsrc<T>(dg, n);
     A4 acc[CB];
@@ -272,30 +281,35 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
     __syncthreads();
 
     const int npan = (n + NB - 1) / NB;
-    for (int p = 0; p < npan; p++) {
+    // Look-ahead schedule. Stage p factors panel p on one wavefront while the
+    // others finish the trailing update of panel p-1; then come panel p's row
+    // interchanges, U12, and the update of column block p+1, which is written
+    // to the other panel buffer. The factoring wavefront is the one that held
+    // the panel rows of p-1: it has no update work left for p-1.
+    {
+        // 1. column block 0 -> panel buffer 0
+        const int l = lane_fresh(), g = l >> 4, jj = l & 15;
+#pragma unroll
+        for (int r = 0; r < 4; r++) S.pnl[0][jj * PR + NB * w + P::crow(g, r)] = acc[0][r];
+    }
+    __syncthreads();
+    int ju_prev = 0;
+    for (int p = 0;; p++) {
         const int j0 = NB * p;
-        const int s0 = p % CB;
-        const int lb = (w - p % RB + RB) % RB;  // local row block of this wavefront
         const int nc = min(NB, n - j0);
+        T* const pn = S.pnl[p & 1];
+        const bool fac = (w == (p + RB - 1) % RB);
 
-        // 1. panel column block -> LDS
-#pragma unroll
-        for (int s = 0; s < CB; s++)
-            if (s == s0) {
-#pragma unroll
-                for (int r = 0; r < 4; r++) S.pnl[jj * PR + NB * lb + P::crow(g, r)] = acc[s][r];
-            }
-        __syncthreads();
-
-        // 2. panel factorisation by wavefront 0, in registers. Lane l keeps
+        // 2. factorisation of panel p in registers. Lane l keeps
         //    window rows l + 64q (q < NQ) as 16-wide vectors; an interchange
         //    only swaps the rows' logical positions, the pivot row is
         //    broadcast with readlane, the rank-1 update runs over the full row
         //    with the pivot-row entries left of the step zeroed, and at the
         //    end every row is stored at its final position (the result of
         //    full-row interchanges).
-        if (w == 0) {
+        if (fac && p < npan) {
             constexpr int NQ = (PR + 63) / 64;
+            const int l = lane_fresh();
 #pragma unroll
             for (int s = 0; s < CB; s++)
 #pragma unroll
@@ -311,10 +325,103 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
                 const int row = l + 64 * q;
                 const bool h = row < PR;
 #pragma unroll
-                for (int k = 0; k < NB; k++) x[q][k] = h ? S.pnl[k * PR + row] : T(0);
+                for (int k = 0; k < NB; k++) x[q][k] = h ? pn[k * PR + row] : T(0);
                 pos[q] = h ? row : kNone;
                 rh[q] = h ? S.rw[(j0 + row) % PR] : T(0);
             }
+            if constexpr (NQ == 2) {
+                // two rows per lane (up to 128 window rows): the same steps
+                // with the two register sets written out, which keeps the
+                // pivot search and broadcast free of per-set branches
+                for (int c = 0; c < nc; c++) {
+                    const int j = j0 + c;
+                    const int km = min(kl, n - 1 - j);
+                    const T xc0 = x[0][c], xc1 = x[1][c];
+                    // candidates: logical rows c..c+km
+                    const bool c0 = (pos[0] >= c) & (pos[0] <= c + km);
+                    const bool c1 = (pos[1] >= c) & (pos[1] <= c + km);
+                    const T v0 = c0 ? fabs(xc0) : T(-1);
+                    const T v1 = c1 ? fabs(xc1) : T(-1);
+                    const T m = P::wmax(fmax(v0, v1));
+                    const bool t0 = c0 & (v0 == m), t1 = c1 & (v1 == m);
+                    const unsigned long long tb0 = __ballot(t0), tb1 = __ballot(t1);
+                    int jpos;
+                    if (__popcll(tb0) + __popcll(tb1) == 1) {
+                        // a unique largest entry (the common case)
+                        jpos = tb0 ? __builtin_amdgcn_readlane(pos[0], __ffsll(tb0) - 1)
+                                   : __builtin_amdgcn_readlane(pos[1], __ffsll(tb1) - 1);
+                    } else if (tb0 | tb1) {
+                        // ties: LAPACK takes the first logical row
+                        jpos = wave_min_int(min(t0 ? pos[0] : kNone, t1 ? pos[1] : kNone));
+                    } else {
+                        jpos = c;
+                    }
+                    const unsigned long long pb0 = __ballot(pos[0] == jpos);
+                    const unsigned long long pb1 = __ballot(pos[1] == jpos);
+                    const int jp = jpos - c;
+                    if (l == 0) ipiv[j] = (int64_t)(j + jp + 1);
+                    const bool in1 = (pb0 == 0);
+                    const int pl = in1 ? (__ffsll(pb1) - 1) : (__ffsll(pb0) - 1);
+                    const T piv = P::readlane(in1 ? xc1 : xc0, pl);
+                    if (piv != T(0)) {
+                        ju = max(ju, min(j + ku + jp, n - 1));
+                        // pivot row right of the step (zero at and left of c)
+                        T pr[NB];
+                        T prh;
+                        if (in1) {
+#pragma unroll
+                            for (int k = 0; k < NB; k++) pr[k] = P::readlane(x[1][k], pl);
+                            prh = P::readlane(rh[1], pl);
+                        } else {
+#pragma unroll
+                            for (int k = 0; k < NB; k++) pr[k] = P::readlane(x[0][k], pl);
+                            prh = P::readlane(rh[0], pl);
+                        }
+#pragma unroll
+                        for (int k = 0; k < NB; k++) pr[k] = (k > c) ? pr[k] : T(0);
+                        // interchange: pivot row -> position c, the row at c -> jpos
+                        pos[0] = (pos[0] == jpos) ? c : ((pos[0] == c) ? jpos : pos[0]);
+                        pos[1] = (pos[1] == jpos) ? c : ((pos[1] == c) ? jpos : pos[1]);
+                        const T rinv = T(1) / piv;
+                        const bool e0 = (pos[0] > c) & (pos[0] <= c + km);
+                        const bool e1 = (pos[1] > c) & (pos[1] <= c + km);
+                        // multipliers; 0 for rows the step does not touch
+                        const T L0 = e0 ? xc0 * rinv : T(0);
+                        const T L1 = e1 ? xc1 * rinv : T(0);
+                        x[0][c] = e0 ? L0 : xc0;
+                        x[1][c] = e1 ? L1 : xc1;
+                        const T tb = -prh;
+                        if (tb != T(0)) {
+                            rh[0] = e0 ? fma(L0, tb, rh[0]) : rh[0];
+                            rh[1] = e1 ? fma(L1, tb, rh[1]) : rh[1];
+                        }
+                        if (fabs(rinv) <= P::maxfinite) {
+                            // finite multipliers: fma(0, t, x) == x for
+                            // untouched rows and fma(L, 0, x) == x for a zero
+                            // pivot-row entry
+#pragma unroll
+                            for (int k = 0; k < NB; k++) {
+                                x[0][k] = fma(L0, -pr[k], x[0][k]);
+                                x[1][k] = fma(L1, -pr[k], x[1][k]);
+                            }
+                        } else {
+#pragma unroll
+                            for (int k = 0; k < NB; k++) {
+                                const T temp = -pr[k];
+                                if (temp != T(0)) {
+                                    x[0][k] = e0 ? fma(L0, temp, x[0][k]) : x[0][k];
+                                    x[1][k] = e1 ? fma(L1, temp, x[1][k]) : x[1][k];
+                                }
+                            }
+                        }
+                        if (l == 0) S.jp[c] = jp;
+                    } else {
+                        if (info == 0) info = j + 1;
+                        if (l == 0) S.jp[c] = 0;
+                    }
+                    if (l == 0 && uext) uext[j] = ju;
+                }
+            } else
             for (int c = 0; c < nc; c++) {
                 const int j = j0 + c;
                 const int km = min(kl, n - 1 - j);
@@ -412,10 +519,7 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
                     if (info == 0) info = j + 1;
                     if (l == 0) S.jp[c] = 0;
                 }
-                if (l == 0) {
-                    S.uext[c] = ju;
-                    if (uext) uext[j] = ju;
-                }
+                if (l == 0 && uext) uext[j] = ju;
             }
             // rows to their final positions, and the sb maps: panel rows keep
             // their index; a row below the panel that moved was pulled up as a
@@ -426,7 +530,7 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
                 const int row = l + 64 * q;
                 if (row < PR) {
 #pragma unroll
-                    for (int k = 0; k < NB; k++) S.pnl[k * PR + pos[q]] = x[q][k];
+                    for (int k = 0; k < NB; k++) pn[k * PR + pos[q]] = x[q][k];
                     S.rw[(j0 + pos[q]) % PR] = rh[q];
                     const int sl = (row < NB) ? row : ((pos[q] != row) ? NB + pos[q] : -1);
                     S.slot[row] = sl;
@@ -443,12 +547,63 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
 #pragma unroll
                 for (int r = 0; r < 4; r++) acc[s][r] = S.stash[(s * 4 + r) * 64 + l];
         }
+        else if (!fac && p > 0) {
+            // 4b. trailing update of panel q = p-1 on the remaining column
+            //     blocks; the finished U rows stream out
+            const int q = p - 1;
+            const int j0 = NB * q, s0 = q % CB, s1 = p % CB, lb = (w - q % RB + RB) % RB, nc = min(NB, n - j0);
+            const T* const pn = S.pnl[q & 1];
+            const int l = lane_fresh(), g = l >> 4, jj = l & 15;
+            const int rowbase = NB * lb;
+            T an[4];
+#pragma unroll
+            for (int kk = 0; kk < 4; kk++) an[kk] = -pn[(4 * kk + g) * PR + rowbase + jj];
+#pragma unroll
+            for (int s = 0; s < CB; s++) {
+                if (s == s0 || s == s1) continue;
+                const int lc = (s - s0 + CB) % CB;
+                if (NB * (q + lc) > ju_prev) continue;
+                A4 c4 = acc[s];
+#pragma unroll
+                for (int kk = 0; kk < 4; kk++) {
+                    const T bv = S.u12[(4 * kk + g) * W + NB * (lc - 1) + jj];
+                    c4 = P::mfma(an[kk], bv, c4);
+                }
+                acc[s] = c4;
+            }
+            // column block q+CB enters the window in slot s0
+            const A4 nt = load_tile<T>(rs, n, nx, q + lb, q + CB, jj, g);
+#pragma unroll
+            for (int s = 0; s < CB; s++)
+                if (s == s0) acc[s] = nt;
+            // U rows j0..j0+nc-1: entry e is row e%16 at column j0 + e/16, up
+            // to kl+ku past the diagonal (the entries past the row's fill
+            // extent are exact zeros in the window). 16 consecutive lanes
+            // write 16 consecutive band rows of one column.
+            {
+                const int c = l & 15;
+                const int r = j0 + c;
+                const int ue = (c < nc) ? min(r + kv, n - 1) : -1;
+                for (int e = 64 * (lb - 1) + l; e < NB * (NB + W); e += 64 * (RB - 1)) {
+                    const int col_off = e >> 4;
+                    const int col = j0 + col_off;
+                    if (col >= r && col <= ue) {
+                        const T v = (col_off < NB) ? pn[col_off * PR + c] : S.u12[c * W + (col_off - NB)];
+                        ab[(size_t)col * ldab + kv + r - col] = v;
+                    }
+                }
+            }
+        }
         __syncthreads();
+        if (p == npan) break;
+        const int s0 = p % CB;
+        const int lb = (w - p % RB + RB) % RB;  // local row block of this wavefront
         const int ju = S.ju;
 
         // 3a. pre-interchange contents of the rows that move -> sb (columns
         //     right of the panel)
         {
+            const int l = lane_fresh(), g = l >> 4, jj = l & 15;
             int sl[4];
 #pragma unroll
             for (int r = 0; r < 4; r++) sl[r] = S.slot[NB * lb + P::crow(g, r)];
@@ -465,11 +620,10 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
             }
         }
         if (lb == 0) {
+            const int l = lane_fresh();
             // the forward-eliminated right-hand side of the panel rows is final
             if (yv)
                 for (int c = l; c < nc; c += 64) yv[j0 + c] = S.rw[(j0 + c) % PR];
-            if (a.write_L)
-                for (int i = l; i < PR * NB; i += 64) S.pnl2[i] = S.pnl[i];
         }
         __syncthreads();
         if (lb == 0) {
@@ -477,21 +631,24 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
             // the window at the bottom for panel p+1 is requested while the
             // other wavefronts form U12, and its right-hand-side entries take
             // the panel rows' rw slots.
+            const int l = lane_fresh(), g = l >> 4, jj = l & 15;
             const int rb = p + RB;
+            // the right-hand-side entry is requested first: vector-memory
+            // loads retire in order, so its wait then covers no tile load
+            const int i = NB * rb + l;
+            const T rv = (l < NB && rhs && i < n) ? rhs[i] : T(0);
 #pragma unroll
             for (int s = 0; s < CB; s++) {
                 const int cb = (p + 1) + ((s - (p + 1) % CB) + CB) % CB;
                 acc[s] = load_tile<T>(rs, n, nx, rb, cb, jj, g);
             }
-            for (int c = l; c < NB; c += 64) {
-                const int i = NB * rb + c;
-                S.rw[i % PR] = (rhs && i < n) ? rhs[i] : T(0);
-            }
+            if (l < NB) S.rw[i % PR] = rv;
         }
 
         // 3b. U12 = L11^-1 P A12, one lane per column, on the wavefronts that
         //     do not hold the panel rows
         if (lb != 0) {
+            const int l = lane_fresh();
             for (int t = NB * 4 * (lb - 1) + l; t < W; t += 64 * (RB - 1)) {
                 T u[NB];
 #pragma unroll
@@ -503,7 +660,7 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
 #pragma unroll
                     for (int m = 0; m < c; m++) {
                         const T temp = -u[m];
-                        if (temp != T(0)) u[c] += S.pnl[m * PR + c] * temp;
+                        if (temp != T(0)) u[c] += pn[m * PR + c] * temp;
                     }
                 }
 #pragma unroll
@@ -512,6 +669,7 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
         }
         // 3c. rows below the panel whose contents changed take them from sb
         if (lb != 0) {
+            const int l = lane_fresh(), g = l >> 4, jj = l & 15;
             int ds[4];
 #pragma unroll
             for (int r = 0; r < 4; r++) ds[r] = S.dslot[NB * lb + P::crow(g, r)];
@@ -529,68 +687,74 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
         }
         __syncthreads();
 
-        // 4. trailing update; the finished U rows stream out
+        // 4a. update of column block p+1 (slot s1), which becomes panel p+1:
+        //     its tiles go to the other panel buffer at their rows in the
+        //     window of p+1 (the wavefront that held the panel rows brings the
+        //     row block that entered the window)
+        const int s1 = (p + 1) % CB;
+        T* const pnx = S.pnl[(p + 1) & 1];
         if (lb != 0) {
-            const int rowbase = NB * lb;
-            T an[4];
+            if (p + 1 < npan) {
+                const int l = lane_fresh(), g = l >> 4, jj = l & 15;
+                if (NB * (p + 1) <= ju) {
+                    T an[4];
 #pragma unroll
-            for (int kk = 0; kk < 4; kk++) an[kk] = -S.pnl[(4 * kk + g) * PR + rowbase + jj];
+                    for (int kk = 0; kk < 4; kk++) an[kk] = -pn[(4 * kk + g) * PR + NB * lb + jj];
 #pragma unroll
-            for (int s = 0; s < CB; s++) {
-                if (s == s0) continue;
-                const int lc = (s - s0 + CB) % CB;
-                if (NB * (p + lc) > ju) continue;
-                A4 c4 = acc[s];
+                    for (int s = 0; s < CB; s++)
+                        if (s == s1) {
+                            A4 c4 = acc[s];
 #pragma unroll
-                for (int kk = 0; kk < 4; kk++) {
-                    const T bv = S.u12[(4 * kk + g) * W + NB * (lc - 1) + jj];
-                    c4 = P::mfma(an[kk], bv, c4);
+                            for (int kk = 0; kk < 4; kk++) c4 = P::mfma(an[kk], S.u12[(4 * kk + g) * W + jj], c4);
+                            acc[s] = c4;
+                        }
                 }
-                acc[s] = c4;
-            }
-            // column block p+CB enters the window in slot s0
-            const A4 nt = load_tile<T>(rs, n, nx, p + lb, p + CB, jj, g);
 #pragma unroll
-            for (int s = 0; s < CB; s++)
-                if (s == s0) acc[s] = nt;
-            // U rows j0..j0+nc-1: entry e is row e%16 at column j0 + e/16, up
-            // to kl+ku past the diagonal (the entries past the row's fill
-            // extent are exact zeros in the window). 16 consecutive lanes
-            // write 16 consecutive band rows of one column.
-            {
-                const int c = l & 15;
-                const int r = j0 + c;
-                const int ue = (c < nc) ? min(r + kv, n - 1) : -1;
-                for (int e = 64 * (lb - 1) + l; e < NB * (NB + W); e += 64 * (RB - 1)) {
-                    const int col_off = e >> 4;
-                    const int col = j0 + col_off;
-                    if (col >= r && col <= ue) {
-                        const T v = (col_off < NB) ? S.pnl[col_off * PR + c] : S.u12[c * W + (col_off - NB)];
-                        ab[(size_t)col * ldab + kv + r - col] = v;
+                for (int s = 0; s < CB; s++)
+                    if (s == s1) {
+#pragma unroll
+                        for (int r = 0; r < 4; r++) pnx[jj * PR + NB * (lb - 1) + P::crow(g, r)] = acc[s][r];
                     }
-                }
             }
-        } else if (a.write_L) {
-            // multipliers in dgbtf2 order: undo the later interchanges of the
-            // panel on the columns left of each pivot step
-            for (int c = nc - 1; c >= 0; c--) {
-                const int jp = S.jp[c];
-                if (jp != 0 && l < c) {
-                    const T t = S.pnl2[l * PR + c];
-                    S.pnl2[l * PR + c] = S.pnl2[l * PR + c + jp];
-                    S.pnl2[l * PR + c + jp] = t;
-                }
+        } else {
+            const int l = lane_fresh(), g = l >> 4, jj = l & 15;
+            if (a.write_L) {
+                // multipliers in dgbtf2 order: undo the later interchanges of
+                // the panel on the columns left of each pivot step, in a copy
+                // of the panel (sb is free until the next panel's row moves)
+                T* const cp = S.sb;
+                for (int i = l; i < PR * NB; i += 64) cp[i] = pn[i];
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                for (int c = nc - 1; c >= 0; c--) {
+                    const int jp = S.jp[c];
+                    if (jp != 0 && l < c) {
+                        const T t = cp[l * PR + c];
+                        cp[l * PR + c] = cp[l * PR + c + jp];
+                        cp[l * PR + c + jp] = t;
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                }
+                for (int c = 0; c < nc; c++) {
+                    const int km = min(kl, n - 1 - (j0 + c));
+                    for (int i = l; i < kl; i += 64)
+                        ab[(size_t)(j0 + c) * ldab + kv + 1 + i] = (i < km) ? cp[c * PR + c + 1 + i] : T(0);
+                }
             }
-            for (int c = 0; c < nc; c++) {
-                const int km = min(kl, n - 1 - (j0 + c));
-                for (int i = l; i < kl; i += 64)
-                    ab[(size_t)(j0 + c) * ldab + kv + 1 + i] = (i < km) ? S.pnl2[c * PR + c + 1 + i] : T(0);
+            if (p + 1 < npan) {
+#pragma unroll
+                for (int s = 0; s < CB; s++)
+                    if (s == s1) {
+#pragma unroll
+                        for (int r = 0; r < 4; r++) pnx[jj * PR + NB * (RB - 1) + P::crow(g, r)] = acc[s][r];
+                    }
             }
         }
         __syncthreads();
+        ju_prev = ju;
     }
     if (tid == 0 && S.info != 0) raise_pivot(a.status, S.info, node);
     __syncthreads();  // the LDS state is reused by the next node
