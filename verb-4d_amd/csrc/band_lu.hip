@@ -32,6 +32,9 @@
 //      skipping column blocks beyond the running fill extent ju; the panel
 //      wavefront meanwhile streams the U rows (and, for the cached path, the
 //      multipliers re-ordered to the dgbtf2 convention) to HBM.
+#include <type_traits>
+#include <utility>
+
 #include "common.h"
 #include "band_lu.h"
 
@@ -125,8 +128,9 @@ struct Prec;
 template <>
 struct Prec<double> {
     typedef double acc4 __attribute__((ext_vector_type(4)));
-    typedef double vec16 __attribute__((ext_vector_type(16)));
     static constexpr double maxfinite = 1.7976931348623157e308;
+    // high word of |v|: orders magnitudes (ties need the low word)
+    __device__ static unsigned key(double v) { return (unsigned)(__double_as_longlong(v) >> 32) & 0x7fffffffu; }
     __device__ static int crow(int g, int r) { return 4 * r + g; }
     __device__ static double readlane(double v, int lane) { return readlane_d(v, lane); }
     __device__ static double wmax(double v) { return wave_max_t(v, [](double x, int q) { return readlane_d(x, q); }); }
@@ -138,8 +142,9 @@ struct Prec<double> {
 template <>
 struct Prec<float> {
     typedef float acc4 __attribute__((ext_vector_type(4)));
-    typedef float vec16 __attribute__((ext_vector_type(16)));
     static constexpr float maxfinite = 3.40282346638528859812e+38f;
+    // bits of |v|: orders magnitudes exactly
+    __device__ static unsigned key(float v) { return (unsigned)__float_as_int(v) & 0x7fffffffu; }
     __device__ static int crow(int g, int r) { return 4 * g + r; }
     __device__ static float readlane(float v, int lane) {
         return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
@@ -190,6 +195,40 @@ __device__ inline int wave_min_int(int v) {
     return __builtin_amdgcn_readfirstlane(v);
 }
 
+// Largest unsigned value over the wavefront: four row shifts (lanes that
+// shift in from outside the row read 0), then lane 15 of each row into the
+// next and lane 31 into rows 2 and 3; lane 63 ends with the maximum.
+__device__ inline unsigned wave_max_u32(unsigned v) {
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, true));  // row_shr:1
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, true));  // row_shr:2
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, true));  // row_shr:4
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, true));  // row_shr:8
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false));  // row_bcast:15
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false));  // row_bcast:31
+    return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+// Calls f(std::integral_constant<int, I>()) for I = B..E-1: the loop index
+// is a compile-time constant from the start, so arrays indexed by it are
+// split into registers before any loop unrolling takes place.
+template <int B, class F, int... I>
+__device__ inline void static_for_seq(F&& f, std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, B + I>()), ...);
+}
+
+template <int B, int E, class F>
+__device__ inline void static_for(F&& f) {
+    if constexpr (E > B) static_for_seq<B>(f, std::make_integer_sequence<int, E - B>());
+}
+
+// Orders this wavefront's LDS accesses before the call against those after
+// it, across lanes (a wavefront executes its LDS instructions in order).
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Lane index read by an opaque instruction sequence. The panel loop takes its
 // lane-dependent indices from this at the start of each phase: computed once
 // outside the loop, the many LDS addresses derived from them stay live across
@@ -230,7 +269,6 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
     using L = Lds<T, RB, CB>;
     using P = Prec<T>;
     using A4 = typename P::acc4;
-    using V16 = typename P::vec16;
     constexpr int W = L::W;
     constexpr int PR = L::PR;
     extern __shared__ double lds_raw[];  // (8-byte aligned)
@@ -309,6 +347,7 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
         //    full-row interchanges).
         if (fac && p < npan) {
             constexpr int NQ = (PR + 63) / 64;
+            static_assert(NQ >= 1 && NQ <= 3, "window rows per lane");
             const int l = lane_fresh();
 #pragma unroll
             for (int s = 0; s < CB; s++)
@@ -317,227 +356,269 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
             int ju = S.ju;
             int info = S.info;
             constexpr int kNone = 1 << 20;
-            V16 x[NQ];
+            // (all indices into these arrays are compile-time constants, so
+            // they live in registers)
+            T x[NQ][NB];
             int pos[NQ];
             T rh[NQ];
-#pragma unroll
-            for (int q = 0; q < NQ; q++) {
+            bool bad = false;
+            static_for<0, NQ>([&](auto qc) {
+                constexpr int q = qc;
                 const int row = l + 64 * q;
                 const bool h = row < PR;
-#pragma unroll
-                for (int k = 0; k < NB; k++) x[q][k] = h ? pn[k * PR + row] : T(0);
+                static_for<0, NB>([&](auto kc) {
+                    constexpr int k = kc;
+                    x[q][k] = h ? pn[k * PR + row] : T(0);
+                    bad |= !isfinite(x[q][k]);
+                });
                 pos[q] = h ? row : kNone;
                 rh[q] = h ? S.rw[(j0 + row) % PR] : T(0);
-            }
-            if constexpr (NQ == 2) {
-                // two rows per lane (up to 128 window rows): the same steps
-                // with the two register sets written out, which keeps the
-                // pivot search and broadcast free of per-set branches
-                for (int c = 0; c < nc; c++) {
-                    const int j = j0 + c;
+                bad |= !isfinite(rh[q]);
+            });
+            // Columns are taken by an unrolled sequence with the column index
+            // known at compile time (only entries right of the step are
+            // touched) while the panel holds only finite values, the largest
+            // candidate is unique in its key and the pivot is nonzero with a
+            // finite reciprocal. Any other column takes the general step
+            // below, and the sequence resumes after it. Per column c, lane c
+            // records the interchange and the U fill extent after the step.
+            const bool fast = __ballot(bad) == 0;
+            int myjp = 0, myju = 0;
+            int c = 0;
+            while (c < nc) {
+                static_for<0, NB>([&](auto Cc) {
+                    constexpr int C = Cc;
+                    if (!(fast && C == c && C < nc)) return;
+                    const int j = j0 + C;
                     const int km = min(kl, n - 1 - j);
-                    const T xc0 = x[0][c], xc1 = x[1][c];
-                    // candidates: logical rows c..c+km
-                    const bool c0 = (pos[0] >= c) & (pos[0] <= c + km);
-                    const bool c1 = (pos[1] >= c) & (pos[1] <= c + km);
-                    const T v0 = c0 ? fabs(xc0) : T(-1);
-                    const T v1 = c1 ? fabs(xc1) : T(-1);
-                    const T m = P::wmax(fmax(v0, v1));
-                    const bool t0 = c0 & (v0 == m), t1 = c1 & (v1 == m);
-                    const unsigned long long tb0 = __ballot(t0), tb1 = __ballot(t1);
-                    int jpos;
-                    if (__popcll(tb0) + __popcll(tb1) == 1) {
-                        // a unique largest entry (the common case)
-                        jpos = tb0 ? __builtin_amdgcn_readlane(pos[0], __ffsll(tb0) - 1)
-                                   : __builtin_amdgcn_readlane(pos[1], __ffsll(tb1) - 1);
-                    } else if (tb0 | tb1) {
-                        // ties: LAPACK takes the first logical row
-                        jpos = wave_min_int(min(t0 ? pos[0] : kNone, t1 ? pos[1] : kNone));
-                    } else {
-                        jpos = c;
-                    }
-                    const unsigned long long pb0 = __ballot(pos[0] == jpos);
-                    const unsigned long long pb1 = __ballot(pos[1] == jpos);
-                    const int jp = jpos - c;
-                    if (l == 0) ipiv[j] = (int64_t)(j + jp + 1);
-                    const bool in1 = (pb0 == 0);
-                    const int pl = in1 ? (__ffsll(pb1) - 1) : (__ffsll(pb0) - 1);
-                    const T piv = P::readlane(in1 ? xc1 : xc0, pl);
-                    if (piv != T(0)) {
-                        ju = max(ju, min(j + ku + jp, n - 1));
-                        // pivot row right of the step (zero at and left of c)
-                        T pr[NB];
-                        T prh;
-                        if (in1) {
-#pragma unroll
-                            for (int k = 0; k < NB; k++) pr[k] = P::readlane(x[1][k], pl);
-                            prh = P::readlane(rh[1], pl);
-                        } else {
-#pragma unroll
-                            for (int k = 0; k < NB; k++) pr[k] = P::readlane(x[0][k], pl);
-                            prh = P::readlane(rh[0], pl);
+                    T xc[NQ];
+                    bool cd[NQ];
+                    unsigned kk[NQ];
+                    unsigned mloc = 0;
+                    static_for<0, NQ>([&](auto qc) {
+                        constexpr int q = qc;
+                        xc[q] = x[q][C];
+                        // candidates: logical rows C..C+km
+                        cd[q] = (pos[q] >= C) & (pos[q] <= C + km);
+                        kk[q] = cd[q] ? P::key(xc[q]) : 0u;
+                        mloc = max(mloc, kk[q]);
+                    });
+                    const unsigned m = wave_max_u32(mloc);
+                    unsigned long long b[NQ];
+                    int cnt = 0;
+                    static_for<0, NQ>([&](auto qc) {
+                        constexpr int q = qc;
+                        b[q] = __ballot(cd[q] & (kk[q] == m));
+                        cnt += __popcll(b[q]);
+                    });
+                    if (cnt != 1) return;
+                    int qp = 0;
+                    unsigned long long bp = b[0];
+                    T xcp = xc[0];
+                    int posp = pos[0];
+                    static_for<1, NQ>([&](auto qc) {
+                        constexpr int q = qc;
+                        const bool here = b[q] != 0;
+                        qp = here ? q : qp;
+                        bp = here ? b[q] : bp;
+                        xcp = here ? xc[q] : xcp;
+                        posp = here ? pos[q] : posp;
+                    });
+                    const int pl = __ffsll(bp) - 1;
+                    const T piv = P::readlane(xcp, pl);
+                    const T rinv = T(1) / piv;
+                    if (!(piv != T(0) && fabs(rinv) <= P::maxfinite)) return;
+                    const int jpos = __builtin_amdgcn_readlane(posp, pl);
+                    const int jp = jpos - C;
+                    ju = max(ju, min(j + ku + jp, n - 1));
+                    myjp = (l == C) ? jp : myjp;
+                    myju = (l == C) ? ju : myju;
+                    // pivot row right of the step (the branches end
+                    // differently so that the compiler keeps them apart
+                    // instead of merging them through a copy of a row set)
+                    T pr[NB];
+                    T prh = T(0);
+                    static_for<0, NQ>([&](auto qc) {
+                        constexpr int q = qc;
+                        if (qp == q) {
+                            static_for<C + 1, NB>([&](auto kc) {
+                                constexpr int k = kc;
+                                pr[k] = P::readlane(x[q][k], pl);
+                            });
+                            prh = P::readlane(rh[q], pl);
+                            asm volatile("; pivot row in set %0" ::"n"(q));
                         }
-#pragma unroll
-                        for (int k = 0; k < NB; k++) pr[k] = (k > c) ? pr[k] : T(0);
-                        // interchange: pivot row -> position c, the row at c -> jpos
-                        pos[0] = (pos[0] == jpos) ? c : ((pos[0] == c) ? jpos : pos[0]);
-                        pos[1] = (pos[1] == jpos) ? c : ((pos[1] == c) ? jpos : pos[1]);
-                        const T rinv = T(1) / piv;
-                        const bool e0 = (pos[0] > c) & (pos[0] <= c + km);
-                        const bool e1 = (pos[1] > c) & (pos[1] <= c + km);
-                        // multipliers; 0 for rows the step does not touch
-                        const T L0 = e0 ? xc0 * rinv : T(0);
-                        const T L1 = e1 ? xc1 * rinv : T(0);
-                        x[0][c] = e0 ? L0 : xc0;
-                        x[1][c] = e1 ? L1 : xc1;
-                        const T tb = -prh;
-                        if (tb != T(0)) {
-                            rh[0] = e0 ? fma(L0, tb, rh[0]) : rh[0];
-                            rh[1] = e1 ? fma(L1, tb, rh[1]) : rh[1];
-                        }
-                        if (fabs(rinv) <= P::maxfinite) {
-                            // finite multipliers: fma(0, t, x) == x for
-                            // untouched rows and fma(L, 0, x) == x for a zero
-                            // pivot-row entry
-#pragma unroll
-                            for (int k = 0; k < NB; k++) {
-                                x[0][k] = fma(L0, -pr[k], x[0][k]);
-                                x[1][k] = fma(L1, -pr[k], x[1][k]);
-                            }
-                        } else {
-#pragma unroll
-                            for (int k = 0; k < NB; k++) {
-                                const T temp = -pr[k];
-                                if (temp != T(0)) {
-                                    x[0][k] = e0 ? fma(L0, temp, x[0][k]) : x[0][k];
-                                    x[1][k] = e1 ? fma(L1, temp, x[1][k]) : x[1][k];
-                                }
-                            }
-                        }
-                        if (l == 0) S.jp[c] = jp;
-                    } else {
-                        if (info == 0) info = j + 1;
-                        if (l == 0) S.jp[c] = 0;
-                    }
-                    if (l == 0 && uext) uext[j] = ju;
-                }
-            } else
-            for (int c = 0; c < nc; c++) {
+                    });
+                    static_for<0, NQ>([&](auto qc) {
+                        constexpr int q = qc;
+                        // interchange: pivot row -> position C, the row at C
+                        // -> jpos; the candidates other than the pivot row
+                        // take multipliers
+                        pos[q] = (pos[q] == jpos) ? C : ((pos[q] == C) ? jpos : pos[q]);
+                        const bool e = cd[q] & (pos[q] != C);
+                        const T Lq = e ? xc[q] * rinv : T(0);
+                        x[q][C] = e ? Lq : xc[q];
+                        // finite values: fma(0, t, v) == v for the rows the
+                        // step does not touch
+                        rh[q] = fma(Lq, -prh, rh[q]);
+                        static_for<C + 1, NB>([&](auto kc) {
+                            constexpr int k = kc;
+                            x[q][k] = fma(Lq, -pr[k], x[q][k]);
+                        });
+                    });
+                    c = C + 1;
+                });
+                if (c >= nc) break;
+
+                // general step for column c, on the rows staged in the panel
+                // buffer at their physical index
+                static_for<0, NQ>([&](auto qc) {
+                    constexpr int q = qc;
+                    const int row = l + 64 * q;
+                    if (row < PR)
+                        static_for<0, NB>([&](auto kc) {
+                            constexpr int k = kc;
+                            pn[k * PR + row] = x[q][k];
+                        });
+                });
+                wave_sync();
                 const int j = j0 + c;
                 const int km = min(kl, n - 1 - j);
                 T xc[NQ], v[NQ];
                 bool cand[NQ];
                 T mloc = T(-1);
-#pragma unroll
-                for (int q = 0; q < NQ; q++) {
-                    xc[q] = x[q][c];
+                static_for<0, NQ>([&](auto qc) {
+                    constexpr int q = qc;
+                    const int row = l + 64 * q;
+                    xc[q] = (row < PR) ? pn[c * PR + row] : T(0);
                     // candidates: logical rows c..c+km
                     cand[q] = (pos[q] >= c) & (pos[q] <= c + km);
                     v[q] = cand[q] ? fabs(xc[q]) : T(-1);
                     mloc = fmax(mloc, v[q]);
-                }
+                });
                 const T m = P::wmax(mloc);
                 unsigned long long tb[NQ];
                 int ntie = 0;
-#pragma unroll
-                for (int q = 0; q < NQ; q++) {
+                static_for<0, NQ>([&](auto qc) {
+                    constexpr int q = qc;
                     tb[q] = __ballot(cand[q] & (v[q] == m));
                     ntie += __popcll(tb[q]);
-                }
+                });
                 int jpos = c;
                 if (ntie == 1) {
-                    // a unique largest entry (the common case)
-#pragma unroll
-                    for (int q = 0; q < NQ; q++)
+                    // a unique largest entry
+                    static_for<0, NQ>([&](auto qc) {
+                        constexpr int q = qc;
                         if (tb[q]) jpos = __builtin_amdgcn_readlane(pos[q], __ffsll(tb[q]) - 1);
+                    });
                 } else if (ntie > 1) {
                     // ties: LAPACK takes the first logical row
                     int mn = kNone;
-#pragma unroll
-                    for (int q = 0; q < NQ; q++) mn = min(mn, (cand[q] & (v[q] == m)) ? pos[q] : kNone);
+                    static_for<0, NQ>([&](auto qc) {
+                        constexpr int q = qc;
+                        mn = min(mn, (cand[q] & (v[q] == m)) ? pos[q] : kNone);
+                    });
                     jpos = wave_min_int(mn);
                 }
                 // the lane and register set holding logical row jpos
                 int qp = 0, pl = 0;
-#pragma unroll
-                for (int q = NQ - 1; q >= 0; q--) {
+                static_for<0, NQ>([&](auto qc) {
+                    constexpr int q = qc;
                     const unsigned long long pb = __ballot(pos[q] == jpos);
                     if (pb) {
                         qp = q;
                         pl = __ffsll(pb) - 1;
                     }
-                }
+                });
                 const int jp = jpos - c;
-                if (l == 0) ipiv[j] = (int64_t)(j + jp + 1);
-                T piv = T(0);
-#pragma unroll
-                for (int q = 0; q < NQ; q++)
-                    if (q == qp) piv = P::readlane(xc[q], pl);
+                myjp = (l == c) ? jp : myjp;
+                T piv = T(0), prh = T(0);
+                static_for<0, NQ>([&](auto qc) {
+                    constexpr int q = qc;
+                    if (q == qp) {
+                        piv = P::readlane(xc[q], pl);
+                        prh = P::readlane(rh[q], pl);
+                    }
+                });
                 if (piv != T(0)) {
                     ju = max(ju, min(j + ku + jp, n - 1));
-                    // pivot row right of the step (zero at and left of c)
-                    T pr[NB];
-                    T prh = T(0);
-#pragma unroll
-                    for (int q = 0; q < NQ; q++) {
-                        if (q == qp) {
-#pragma unroll
-                            for (int k = 0; k < NB; k++) pr[k] = P::readlane(x[q][k], pl);
-                            prh = P::readlane(rh[q], pl);
-                        }
-                    }
-#pragma unroll
-                    for (int k = 0; k < NB; k++) pr[k] = (k > c) ? pr[k] : T(0);
+                    const int prow = pl + 64 * qp;
                     const T rinv = T(1) / piv;
                     const T tb2 = -prh;
                     const bool finite = fabs(rinv) <= P::maxfinite;
-#pragma unroll
-                    for (int q = 0; q < NQ; q++) {
+                    T Lv[NQ];
+                    bool ev[NQ];
+                    static_for<0, NQ>([&](auto qc) {
+                        constexpr int q = qc;
                         // interchange: pivot row -> position c, the row at c -> jpos
                         pos[q] = (pos[q] == jpos) ? c : ((pos[q] == c) ? jpos : pos[q]);
                         const bool e = (pos[q] > c) & (pos[q] <= c + km);
                         // multipliers; 0 for rows the step does not touch
                         const T Lq = e ? xc[q] * rinv : T(0);
-                        x[q][c] = e ? Lq : xc[q];
+                        const int row = l + 64 * q;
+                        if (row < PR) pn[c * PR + row] = e ? Lq : xc[q];
                         if (tb2 != T(0)) rh[q] = e ? fma(Lq, tb2, rh[q]) : rh[q];
-                        if (finite) {
-                            // finite multipliers: fma(0, t, x) == x for
-                            // untouched rows and fma(L, 0, x) == x for a zero
-                            // pivot-row entry
-#pragma unroll
-                            for (int k = 0; k < NB; k++) x[q][k] = fma(Lq, -pr[k], x[q][k]);
-                        } else {
-#pragma unroll
-                            for (int k = 0; k < NB; k++) {
-                                const T temp = -pr[k];
-                                if (temp != T(0)) x[q][k] = e ? fma(Lq, temp, x[q][k]) : x[q][k];
+                        Lv[q] = Lq;
+                        ev[q] = e;
+                    });
+                    // rank-1 update right of the step; the pivot row's own
+                    // entries stay as they are (its multiplier is 0)
+                    for (int k = c + 1; k < NB; k++) {
+                        const T t = -pn[k * PR + prow];
+                        static_for<0, NQ>([&](auto qc) {
+                            constexpr int q = qc;
+                            const int row = l + 64 * q;
+                            if (row < PR) {
+                                const T xv = pn[k * PR + row];
+                                // finite multipliers: fma(0, t, x) == x for
+                                // untouched rows and fma(L, 0, x) == x for a
+                                // zero pivot-row entry
+                                pn[k * PR + row] = (finite || (t != T(0) && ev[q])) ? fma(Lv[q], t, xv) : xv;
                             }
-                        }
+                        });
                     }
-                    if (l == 0) S.jp[c] = jp;
                 } else {
                     if (info == 0) info = j + 1;
-                    if (l == 0) S.jp[c] = 0;
                 }
-                if (l == 0 && uext) uext[j] = ju;
+                myju = (l == c) ? ju : myju;
+                wave_sync();
+                static_for<0, NQ>([&](auto qc) {
+                    constexpr int q = qc;
+                    const int row = l + 64 * q;
+                    if (row < PR)
+                        static_for<0, NB>([&](auto kc) {
+                            constexpr int k = kc;
+                            x[q][k] = pn[k * PR + row];
+                        });
+                });
+                wave_sync();
+                c++;
             }
             // rows to their final positions, and the sb maps: panel rows keep
             // their index; a row below the panel that moved was pulled up as a
             // pivot (at most one per column) and takes sb row 16 + its final
             // position. dslot gives each window row the sb row it takes.
-#pragma unroll
-            for (int q = 0; q < NQ; q++) {
+            static_for<0, NQ>([&](auto qc) {
+                constexpr int q = qc;
                 const int row = l + 64 * q;
                 if (row < PR) {
-#pragma unroll
-                    for (int k = 0; k < NB; k++) pn[k * PR + pos[q]] = x[q][k];
+                    static_for<0, NB>([&](auto kc) {
+                        constexpr int k = kc;
+                        pn[k * PR + pos[q]] = x[q][k];
+                    });
                     S.rw[(j0 + pos[q]) % PR] = rh[q];
                     const int sl = (row < NB) ? row : ((pos[q] != row) ? NB + pos[q] : -1);
                     S.slot[row] = sl;
                     S.dslot[pos[q]] = (pos[q] < NB || pos[q] != row) ? sl : -1;
                 }
+            });
+            if (l < NB) {
+                S.jp[l] = (l < nc) ? myjp : 0;
+                if (l < nc) {
+                    ipiv[j0 + l] = (int64_t)(j0 + l + myjp + 1);
+                    if (uext) uext[j0 + l] = myju;
+                }
             }
-            if (l < NB && l >= nc) S.jp[l] = 0;
             if (l == 0) {
                 S.ju = ju;
                 S.info = info;
@@ -555,6 +636,15 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
             const T* const pn = S.pnl[q & 1];
             const int l = lane_fresh(), g = l >> 4, jj = l & 15;
             const int rowbase = NB * lb;
+            // column block q+CB enters the window in slot s0 (requested first:
+            // the wavefront that holds the next panel rows stores this tile
+            // right after the barrier)
+            {
+                const A4 nt = load_tile<T>(rs, n, nx, q + lb, q + CB, jj, g);
+#pragma unroll
+                for (int s = 0; s < CB; s++)
+                    if (s == s0) acc[s] = nt;
+            }
             T an[4];
 #pragma unroll
             for (int kk = 0; kk < 4; kk++) an[kk] = -pn[(4 * kk + g) * PR + rowbase + jj];
@@ -571,11 +661,6 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
                 }
                 acc[s] = c4;
             }
-            // column block q+CB enters the window in slot s0
-            const A4 nt = load_tile<T>(rs, n, nx, q + lb, q + CB, jj, g);
-#pragma unroll
-            for (int s = 0; s < CB; s++)
-                if (s == s0) acc[s] = nt;
             // U rows j0..j0+nc-1: entry e is row e%16 at column j0 + e/16, up
             // to kl+ku past the diagonal (the entries past the row's fill
             // extent are exact zeros in the window). 16 consecutive lanes
@@ -651,20 +736,26 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
             const int l = lane_fresh();
             for (int t = NB * 4 * (lb - 1) + l; t < W; t += 64 * (RB - 1)) {
                 T u[NB];
-#pragma unroll
-                for (int c = 0; c < NB; c++) u[c] = (c < nc) ? S.sb[S.dslot[c] * W + t] : T(0);
-#pragma unroll
-                for (int c = 1; c < NB; c++) {
-                    // one row of L11 at a time keeps the register footprint small
-                    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                    for (int m = 0; m < c; m++) {
-                        const T temp = -u[m];
-                        if (temp != T(0)) u[c] += pn[m * PR + c] * temp;
-                    }
-                }
-#pragma unroll
-                for (int c = 0; c < NB; c++) S.u12[c * W + t] = u[c];
+                static_for<0, NB>([&](auto cc) {
+                    constexpr int c = cc;
+                    u[c] = (c < nc) ? S.sb[S.dslot[c] * W + t] : T(0);
+                });
+                // column-oriented, as the reference TRSM: once u[m] is final
+                // it updates every later row (a zero u[m] is skipped), so the
+                // rows' updates are independent of each other
+                static_for<0, NB - 1>([&](auto mc) {
+                    constexpr int m = mc;
+                    const T temp = -u[m];
+                    if (temp != T(0))
+                        static_for<m + 1, NB>([&](auto cc) {
+                            constexpr int c = cc;
+                            u[c] = fma(pn[m * PR + c], temp, u[c]);
+                        });
+                });
+                static_for<0, NB>([&](auto cc) {
+                    constexpr int c = cc;
+                    S.u12[c * W + t] = u[c];
+                });
             }
         }
         // 3c. rows below the panel whose contents changed take them from sb
