@@ -271,6 +271,7 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
     using A4 = typename P::acc4;
     constexpr int W = L::W;
     constexpr int PR = L::PR;
+    static_assert(RB >= 3, "the trailing update's U rows are written by row blocks 2..RB-1");
     extern __shared__ double lds_raw[];  // (8-byte aligned)
     L& S = *reinterpret_cast<L*>(lds_raw);
 
@@ -632,44 +633,48 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
             // 4b. trailing update of panel q = p-1 on the remaining column
             //     blocks; the finished U rows stream out
             const int q = p - 1;
-            const int j0 = NB * q, s0 = q % CB, s1 = p % CB, lb = (w - q % RB + RB) % RB, nc = min(NB, n - j0);
+            const int j0 = NB * q, lb = (w - q % RB + RB) % RB, nc = min(NB, n - j0);
             const T* const pn = S.pnl[q & 1];
             const int l = lane_fresh(), g = l >> 4, jj = l & 15;
             const int rowbase = NB * lb;
-            // column block q+CB enters the window in slot s0 (requested first:
-            // the wavefront that holds the next panel rows stores this tile
-            // right after the barrier)
-            {
-                const A4 nt = load_tile<T>(rs, n, nx, q + lb, q + CB, jj, g);
-#pragma unroll
-                for (int s = 0; s < CB; s++)
-                    if (s == s0) acc[s] = nt;
-            }
+            // column block q+CB enters the window in slot 0, whose panel tile
+            // is done with (requested first: the wavefront that holds the next
+            // panel rows stores this tile right after the barrier)
+            acc[0] = load_tile<T>(rs, n, nx, q + lb, q + CB, jj, g);
             T an[4];
 #pragma unroll
             for (int kk = 0; kk < 4; kk++) an[kk] = -pn[(4 * kk + g) * PR + rowbase + jj];
+            // slot 1 was updated in step 4a
 #pragma unroll
-            for (int s = 0; s < CB; s++) {
-                if (s == s0 || s == s1) continue;
-                const int lc = (s - s0 + CB) % CB;
-                if (NB * (q + lc) > ju_prev) continue;
+            for (int s = 2; s < CB; s++) {
+                if (NB * (q + s) > ju_prev) continue;
                 A4 c4 = acc[s];
 #pragma unroll
                 for (int kk = 0; kk < 4; kk++) {
-                    const T bv = S.u12[(4 * kk + g) * W + NB * (lc - 1) + jj];
+                    const T bv = S.u12[(4 * kk + g) * W + NB * (s - 1) + jj];
                     c4 = P::mfma(an[kk], bv, c4);
                 }
                 acc[s] = c4;
             }
+            // slot s holds column block p + s again
+            {
+                const A4 t0 = acc[0];
+#pragma unroll
+                for (int s = 0; s + 1 < CB; s++) acc[s] = acc[s + 1];
+                acc[CB - 1] = t0;
+            }
             // U rows j0..j0+nc-1: entry e is row e%16 at column j0 + e/16, up
             // to kl+ku past the diagonal (the entries past the row's fill
             // extent are exact zeros in the window). 16 consecutive lanes
-            // write 16 consecutive band rows of one column.
-            {
+            // write 16 consecutive band rows of one column. The wavefront
+            // that holds the next panel rows writes none: it waits for its
+            // entering tile right after the barrier, and that wait would
+            // cover these stores.
+            if (lb >= 2) {
                 const int c = l & 15;
                 const int r = j0 + c;
                 const int ue = (c < nc) ? min(r + kv, n - 1) : -1;
-                for (int e = 64 * (lb - 1) + l; e < NB * (NB + W); e += 64 * (RB - 1)) {
+                for (int e = 64 * (lb - 2) + l; e < NB * (NB + W); e += 64 * (RB - 2)) {
                     const int col_off = e >> 4;
                     const int col = j0 + col_off;
                     if (col >= r && col <= ue) {
@@ -681,7 +686,6 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
         }
         __syncthreads();
         if (p == npan) break;
-        const int s0 = p % CB;
         const int lb = (w - p % RB + RB) % RB;  // local row block of this wavefront
         const int ju = S.ju;
 
@@ -696,11 +700,7 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
             for (int r = 0; r < 4; r++) {
                 if (sl[r] >= 0) {
 #pragma unroll
-                    for (int s = 0; s < CB; s++) {
-                        if (s == s0) continue;
-                        const int lc = (s - s0 + CB) % CB;
-                        S.sb[sl[r] * W + NB * (lc - 1) + jj] = acc[s][r];
-                    }
+                    for (int s = 1; s < CB; s++) S.sb[sl[r] * W + NB * (s - 1) + jj] = acc[s][r];
                 }
             }
         }
@@ -711,23 +711,21 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
                 for (int c = l; c < nc; c += 64) yv[j0 + c] = S.rw[(j0 + c) % PR];
         }
         __syncthreads();
+        // right-hand-side entry of the row block that enters the window
+        T rv = T(0);
         if (lb == 0) {
             // The panel rows' registers are free: the row block that enters
             // the window at the bottom for panel p+1 is requested while the
-            // other wavefronts form U12, and its right-hand-side entries take
-            // the panel rows' rw slots.
+            // other wavefronts form U12 (nothing waits for these loads before
+            // step 4a, where its right-hand-side entries take the panel rows'
+            // rw slots).
             const int l = lane_fresh(), g = l >> 4, jj = l & 15;
             const int rb = p + RB;
-            // the right-hand-side entry is requested first: vector-memory
-            // loads retire in order, so its wait then covers no tile load
             const int i = NB * rb + l;
-            const T rv = (l < NB && rhs && i < n) ? rhs[i] : T(0);
+            rv = (l < NB && rhs && i < n) ? rhs[i] : T(0);
+            // slot s: column block p + 1 + s (the layout of panel p+1)
 #pragma unroll
-            for (int s = 0; s < CB; s++) {
-                const int cb = (p + 1) + ((s - (p + 1) % CB) + CB) % CB;
-                acc[s] = load_tile<T>(rs, n, nx, rb, cb, jj, g);
-            }
-            if (l < NB) S.rw[i % PR] = rv;
+            for (int s = 0; s < CB; s++) acc[s] = load_tile<T>(rs, n, nx, rb, p + 1 + s, jj, g);
         }
 
         // 3b. U12 = L11^-1 P A12, one lane per column, on the wavefronts that
@@ -768,21 +766,16 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
             for (int r = 0; r < 4; r++) {
                 if (ds[r] >= 0) {
 #pragma unroll
-                    for (int s = 0; s < CB; s++) {
-                        if (s == s0) continue;
-                        const int lc = (s - s0 + CB) % CB;
-                        acc[s][r] = S.sb[ds[r] * W + NB * (lc - 1) + jj];
-                    }
+                    for (int s = 1; s < CB; s++) acc[s][r] = S.sb[ds[r] * W + NB * (s - 1) + jj];
                 }
             }
         }
         __syncthreads();
 
-        // 4a. update of column block p+1 (slot s1), which becomes panel p+1:
+        // 4a. update of column block p+1 (slot 1), which becomes panel p+1:
         //     its tiles go to the other panel buffer at their rows in the
         //     window of p+1 (the wavefront that held the panel rows brings the
-        //     row block that entered the window)
-        const int s1 = (p + 1) % CB;
+        //     row block that entered the window, in slot 0 of its new layout)
         T* const pnx = S.pnl[(p + 1) & 1];
         if (lb != 0) {
             if (p + 1 < npan) {
@@ -791,21 +784,13 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
                     T an[4];
 #pragma unroll
                     for (int kk = 0; kk < 4; kk++) an[kk] = -pn[(4 * kk + g) * PR + NB * lb + jj];
+                    A4 c4 = acc[1];
 #pragma unroll
-                    for (int s = 0; s < CB; s++)
-                        if (s == s1) {
-                            A4 c4 = acc[s];
-#pragma unroll
-                            for (int kk = 0; kk < 4; kk++) c4 = P::mfma(an[kk], S.u12[(4 * kk + g) * W + jj], c4);
-                            acc[s] = c4;
-                        }
+                    for (int kk = 0; kk < 4; kk++) c4 = P::mfma(an[kk], S.u12[(4 * kk + g) * W + jj], c4);
+                    acc[1] = c4;
                 }
 #pragma unroll
-                for (int s = 0; s < CB; s++)
-                    if (s == s1) {
-#pragma unroll
-                        for (int r = 0; r < 4; r++) pnx[jj * PR + NB * (lb - 1) + P::crow(g, r)] = acc[s][r];
-                    }
+                for (int r = 0; r < 4; r++) pnx[jj * PR + NB * (lb - 1) + P::crow(g, r)] = acc[1][r];
             }
         } else {
             const int l = lane_fresh(), g = l >> 4, jj = l & 15;
@@ -835,13 +820,10 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
                         ab[(size_t)(j0 + c) * ldab + kv + 1 + i] = (i < km) ? cp[c * PR + c + 1 + i] : T(0);
                 }
             }
+            if (l < NB) S.rw[(NB * (p + RB) + l) % PR] = rv;
             if (p + 1 < npan) {
 #pragma unroll
-                for (int s = 0; s < CB; s++)
-                    if (s == s1) {
-#pragma unroll
-                        for (int r = 0; r < 4; r++) pnx[jj * PR + NB * (RB - 1) + P::crow(g, r)] = acc[s][r];
-                    }
+                for (int r = 0; r < 4; r++) pnx[jj * PR + NB * (RB - 1) + P::crow(g, r)] = acc[0][r];
             }
         }
         __syncthreads();
