@@ -253,10 +253,12 @@ struct Lds {
     T sb[2 * NB * W];                        // pre-interchange contents of the rows that move
     T stash[CB * 4 * 64];                    // wavefront 0's accumulators while it factors
     T rw[PR];                                // right-hand-side window (index mod PR)
+    T linv[NB * NB];                         // L11^-1 of the current panel, row-major
     int slot[PR];                            // sb row holding window row i's old contents, -1: none
     int dslot[PR];                           // sb row whose contents window row i takes, -1: unchanged
     int jp[NB];
     int ju, info;
+    int lfin;                                // the panel's multipliers are all finite
     int qnode;                               // persistent mode: current dispatch slot
 };
 
@@ -326,10 +328,16 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
     // to the other panel buffer. The factoring wavefront is the one that held
     // the panel rows of p-1: it has no update work left for p-1.
     {
-        // 1. column block 0 -> panel buffer 0
+        // 1. column block 0 -> panel buffer 0; the wavefront that factors
+        //    panel 0 keeps its other tiles in the stash meanwhile
         const int l = lane_fresh(), g = l >> 4, jj = l & 15;
 #pragma unroll
         for (int r = 0; r < 4; r++) S.pnl[0][jj * PR + NB * w + P::crow(g, r)] = acc[0][r];
+        if (w == RB - 1)
+#pragma unroll
+            for (int s = 1; s < CB; s++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) S.stash[(s * 4 + r) * 64 + l] = acc[s][r];
     }
     __syncthreads();
     int ju_prev = 0;
@@ -350,10 +358,10 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
             constexpr int NQ = (PR + 63) / 64;
             static_assert(NQ >= 1 && NQ <= 3, "window rows per lane");
             const int l = lane_fresh();
+            // this wavefront's tiles are dead here: slot 0 is the panel, the
+            // others come from the stash after the stage barrier
 #pragma unroll
-            for (int s = 0; s < CB; s++)
-#pragma unroll
-                for (int r = 0; r < 4; r++) S.stash[(s * 4 + r) * 64 + l] = acc[s][r];
+            for (int s = 0; s < CB; s++) acc[s] = __builtin_nondeterministic_value(acc[s]);
             int ju = S.ju;
             int info = S.info;
             constexpr int kNone = 1 << 20;
@@ -384,6 +392,7 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
             // below, and the sequence resumes after it. Per column c, lane c
             // records the interchange and the U fill extent after the step.
             const bool fast = __ballot(bad) == 0;
+            bool nonfin = !fast;  // (multipliers may be non-finite)
             int myjp = 0, myju = 0;
             int c = 0;
             while (c < nc) {
@@ -547,6 +556,7 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
                     const T rinv = T(1) / piv;
                     const T tb2 = -prh;
                     const bool finite = fabs(rinv) <= P::maxfinite;
+                    nonfin |= !finite;
                     T Lv[NQ];
                     bool ev[NQ];
                     static_for<0, NQ>([&](auto qc) {
@@ -623,11 +633,8 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
             if (l == 0) {
                 S.ju = ju;
                 S.info = info;
+                S.lfin = !nonfin;
             }
-#pragma unroll
-            for (int s = 0; s < CB; s++)
-#pragma unroll
-                for (int r = 0; r < 4; r++) acc[s][r] = S.stash[(s * 4 + r) * 64 + l];
         }
         else if (!fac && p > 0) {
             // 4b. trailing update of panel q = p-1 on the remaining column
@@ -641,6 +648,16 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
             // is done with (requested first: the wavefront that holds the next
             // panel rows stores this tile right after the barrier)
             acc[0] = load_tile<T>(rs, n, nx, q + lb, q + CB, jj, g);
+            // The factoring wavefront's row block (p+RB-1, which entered the
+            // window at panel p) in slots 1..CB-1: these wavefronts request
+            // its tiles and leave them in the stash.
+            constexpr int NT = (CB - 1 + RB - 2) / (RB - 1);
+            A4 ft[NT];
+            static_for<0, NT>([&](auto tc) {
+                constexpr int t = tc;
+                const int sft = 1 + (lb - 1) + t * (RB - 1);
+                if (p < npan && sft < CB) ft[t] = load_tile<T>(rs, n, nx, p + RB - 1, p + sft, jj, g);
+            });
             T an[4];
 #pragma unroll
             for (int kk = 0; kk < 4; kk++) an[kk] = -pn[(4 * kk + g) * PR + rowbase + jj];
@@ -663,6 +680,13 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
                 for (int s = 0; s + 1 < CB; s++) acc[s] = acc[s + 1];
                 acc[CB - 1] = t0;
             }
+            static_for<0, NT>([&](auto tc) {
+                constexpr int t = tc;
+                const int sft = 1 + (lb - 1) + t * (RB - 1);
+                if (p < npan && sft < CB)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) S.stash[(sft * 4 + r) * 64 + l] = ft[t][r];
+            });
             // U rows j0..j0+nc-1: entry e is row e%16 at column j0 + e/16, up
             // to kl+ku past the diagonal (the entries past the row's fill
             // extent are exact zeros in the window). 16 consecutive lanes
@@ -688,6 +712,63 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
         if (p == npan) break;
         const int lb = (w - p % RB + RB) % RB;  // local row block of this wavefront
         const int ju = S.ju;
+        const bool lfin = S.lfin != 0;
+        if (fac) {
+            const int l = lane_fresh();
+            // L11^-1 for U12 below (finite multipliers): lane m solves
+            // L11 y = e_m column-wise; the other wavefronts are busier here.
+            // (Before this wavefront's tiles come back from the stash, so
+            // that the L11 entries can all be requested at once.)
+            if (lfin && l < NB) {
+                T y[NB];
+                static_for<0, NB>([&](auto cc) {
+                    constexpr int c = cc;
+                    y[c] = (c == l) ? T(1) : T(0);
+                });
+                // the L11 entries of columns [I0, I1) are requested together
+                // (one wait per group instead of one per column)
+                T lc[NB * NB];
+                auto fetch = [&](auto i0c, auto i1c) {
+                    static_for<decltype(i0c)::value, decltype(i1c)::value>([&](auto ic) {
+                        constexpr int i = ic;
+                        static_for<i + 1, NB>([&](auto cc) {
+                            constexpr int c = cc;
+                            lc[i * NB + c] = pn[i * PR + c];
+                        });
+                    });
+                };
+                auto eliminate = [&](auto i0c, auto i1c) {
+                    static_for<decltype(i0c)::value, decltype(i1c)::value>([&](auto ic) {
+                        constexpr int i = ic;
+                        const T yi = y[i];
+                        static_for<i + 1, NB>([&](auto cc) {
+                            constexpr int c = cc;
+                            y[c] = fma(-lc[i * NB + c], yi, y[c]);
+                        });
+                    });
+                };
+                using I0 = std::integral_constant<int, 0>;
+                using I3 = std::integral_constant<int, 3>;
+                using I7 = std::integral_constant<int, 7>;
+                using IE = std::integral_constant<int, NB - 1>;
+                fetch(I0(), I3());
+                fetch(I3(), I7());
+                __builtin_amdgcn_sched_barrier(0);
+                eliminate(I0(), I3());
+                fetch(I7(), IE());
+                __builtin_amdgcn_sched_barrier(0);
+                eliminate(I3(), I7());
+                eliminate(I7(), IE());
+                static_for<0, NB>([&](auto cc) {
+                    constexpr int c = cc;
+                    S.linv[c * NB + l] = y[c];
+                });
+            }
+#pragma unroll
+            for (int s = 1; s < CB; s++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) acc[s][r] = S.stash[(s * 4 + r) * 64 + l];
+        }
 
         // 3a. pre-interchange contents of the rows that move -> sb (columns
         //     right of the panel)
@@ -715,22 +796,44 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
         T rv = T(0);
         if (lb == 0) {
             // The panel rows' registers are free: the row block that enters
-            // the window at the bottom for panel p+1 is requested while the
-            // other wavefronts form U12 (nothing waits for these loads before
-            // step 4a, where its right-hand-side entries take the panel rows'
-            // rw slots).
+            // the window at the bottom for panel p+1 belongs to this
+            // wavefront. Its tile in column block p+1 and its right-hand-side
+            // entries are requested while the other wavefronts form U12
+            // (nothing waits for them before step 4a, where the entries take
+            // the panel rows' rw slots).
             const int l = lane_fresh(), g = l >> 4, jj = l & 15;
             const int rb = p + RB;
             const int i = NB * rb + l;
             rv = (l < NB && rhs && i < n) ? rhs[i] : T(0);
-            // slot s: column block p + 1 + s (the layout of panel p+1)
-#pragma unroll
-            for (int s = 0; s < CB; s++) acc[s] = load_tile<T>(rs, n, nx, rb, p + 1 + s, jj, g);
+            // slot 0 of the layout of panel p+1 (column block p+1); the
+            // other slots come through the stash while panel p+1 is factored
+            acc[0] = load_tile<T>(rs, n, nx, rb, p + 1, jj, g);
         }
 
-        // 3b. U12 = L11^-1 P A12, one lane per column, on the wavefronts that
-        //     do not hold the panel rows
-        if (lb != 0) {
+        // 3b. U12 = L11^-1 P A12 on the wavefronts that do not hold the panel
+        //     rows: with finite multipliers as L11^-1 times the moved rows,
+        //     one 16x16 tile per MFMA chain (column blocks dealt out to the
+        //     wavefronts); otherwise by substitution, one lane per column
+        if (lb != 0 && lfin) {
+            const int l = lane_fresh(), g = l >> 4, jj = l & 15;
+            T la[4];
+            int dsl[4];
+#pragma unroll
+            for (int kk = 0; kk < 4; kk++) {
+                la[kk] = S.linv[jj * NB + 4 * kk + g];
+                dsl[kk] = (4 * kk + g < nc) ? S.dslot[4 * kk + g] : -1;
+            }
+            for (int cb = lb - 1; cb < CB - 1; cb += RB - 1) {
+                A4 d = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+                for (int kk = 0; kk < 4; kk++) {
+                    const T bv = (dsl[kk] >= 0) ? S.sb[dsl[kk] * W + NB * cb + jj] : T(0);
+                    d = P::mfma(la[kk], bv, d);
+                }
+#pragma unroll
+                for (int r = 0; r < 4; r++) S.u12[P::crow(g, r) * W + NB * cb + jj] = d[r];
+            }
+        } else if (lb != 0) {
             const int l = lane_fresh();
             for (int t = NB * 4 * (lb - 1) + l; t < W; t += 64 * (RB - 1)) {
                 T u[NB];
