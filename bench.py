@@ -147,6 +147,8 @@ def main():
     ap.add_argument("--no-gmres", action="store_true",
                     help="skip timing steps whose local solve is the matrix-free GMRES path")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--tuning", default=None,
+                    help="fixed launch shapes TQ,RB,CB,PERSISTENT instead of the autotuner (profiling runs)")
     ap.add_argument("--no-uneven", action="store_true",
                     help="skip the uneven-extent production grid (loss-cone pitch-angle extents, 4040..9191 unknowns)")
     ap.add_argument("--no-graph", action="store_true",
@@ -182,7 +184,11 @@ def main():
     torch.cuda.synchronize()
     ctx.synchronize()
     tuning = None
-    if not args.no_autotune:
+    if args.tuning:
+        tq, rb, cb, pers = (int(v) for v in args.tuning.split(","))
+        dg.set_tuning(convection_tq=tq, lu_rb=rb, lu_cb=cb, lu_persistent=pers)
+        tuning = {"convection_tq": tq, "lu_rb": rb, "lu_cb": cb, "lu_persistent": pers, "fixed": True}
+    elif not args.no_autotune:
         tuning = dg.autotune()  # launch shapes only; psd is restored
         print("bench: autotuned %s" % tuning, file=sys.stderr, flush=True)
 
