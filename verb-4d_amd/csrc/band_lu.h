@@ -22,7 +22,25 @@ struct LuArgsT {
     const int* done = nullptr; // nodes with done[node] != 0 are left alone (null: none)
     int* queue = nullptr;      // persistent mode: workgroups take dispatch slots 0..qnodes-1 from this counter (zeroed)
     int qnodes = 0;
+    // U in 16-row blocks instead of the LAPACK layout (write_L == 0 only):
+    // block b of a node holds rows 16b..16b+15 and the ublk_cols columns
+    // from 16b on, column-major (16 consecutive rows per column), at
+    // ab + node*ublk_stride + b*16*ublk_cols; entries outside the band are 0
+    int ublk_cols = 0;         // 0: LAPACK layout
+    size_t ublk_stride = 0;
 };
+
+// Blocked U geometry for n unknowns and bandwidths kl, ku: columns per block
+// (kl+ku+16 rounded up to 16) and the per-node stride. Returns false when the
+// blocked factors would not fit in the LAPACK band storage of the node.
+inline bool ublk_geometry(int n, int kl, int ku, int* cols, size_t* stride) {
+    const int c = (kl + ku + 16 + 15) / 16 * 16;
+    const size_t st = (size_t)((n + 15) / 16) * 16 * c;
+    if (st > (size_t)(2 * kl + ku + 1) * n) return false;
+    *cols = c;
+    *stride = st;
+    return true;
+}
 
 using LuArgs = LuArgsT<double>;
 

@@ -694,7 +694,22 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
             // that holds the next panel rows writes none: it waits for its
             // entering tile right after the barrier, and that wait would
             // cover these stores.
-            if (lb >= 2) {
+            if (lb >= 2 && a.ublk_cols) {
+                // blocked U: the whole 16 x ublk_cols block, zeros outside
+                // the band, in one contiguous stretch
+                const int c = l & 15;
+                const int r = j0 + c;
+                const int ue = (c < nc) ? min(r + kv, n - 1) : -1;
+                const int wc = a.ublk_cols;
+                T* dst = a.ab + (size_t)node * a.ublk_stride + (size_t)q * NB * wc;
+                for (int e = 64 * (lb - 2) + l; e < NB * wc; e += 64 * (RB - 2)) {
+                    const int col_off = e >> 4;
+                    const int col = j0 + col_off;
+                    T v = T(0);
+                    if (col >= r && col <= ue) v = (col_off < NB) ? pn[col_off * PR + c] : S.u12[c * W + (col_off - NB)];
+                    dst[e] = v;
+                }
+            } else if (lb >= 2) {
                 const int c = l & 15;
                 const int r = j0 + c;
                 const int ue = (c < nc) ? min(r + kv, n - 1) : -1;
@@ -940,84 +955,94 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
 // ---------------------------------------------------------------------------
 // x = U^-1 y, one workgroup per node. U is read in blocks of 16 rows x
 // (kl+ku+16) columns (the factorisation stores every U row up to kl+ku past
-// its diagonal, zeros included), with buffer loads issued one block ahead
-// into registers and written to LDS after the current block is solved; x
-// is kept in LDS only over the band ahead of the current block (a ring of
-// kXRing entries; the solution streams to HBM), so two nodes fit in one CU.
-// Within a block: the part of each row right of the block is reduced by 16
-// threads per row, then one wavefront solves the 16x16 triangle last row
-// first, broadcasting each x with readlane.
+// its diagonal, zeros included) into three register sets used in turn, with
+// buffer loads issued two blocks ahead; x is kept in LDS only over the band
+// ahead of the current block (a ring of kXRing entries; the solution streams
+// to HBM). Within a block: each of the 16 threads of a row reduces its
+// columns right of the block straight from registers, then one wavefront
+// solves the 16x16 triangle (staged in LDS) last row first, broadcasting
+// each x with readlane.
 // ---------------------------------------------------------------------------
 constexpr int kBsThreads = 256;
 constexpr int kBsMaxI = 20;  // staged 16-column groups per thread: (kl+ku+16)/16 <= 20
 constexpr int kXRing = 512;  // x entries kept in LDS: >= kl+ku+32, a power of two
 
-template <typename T>
+template <typename T, bool BLK, int NI, int D>
 __global__ __launch_bounds__(kBsThreads) void band_backsolve(const T* __restrict__ ab_all,
                                                              const T* __restrict__ y_all,
                                                              T* __restrict__ x_all, int n, int kl, int ku,
                                                              int ldab, int nR, int include_boundary,
                                                              const int* __restrict__ nn,
                                                              const int* __restrict__ order,
-                                                             const int* __restrict__ done) {
+                                                             const int* __restrict__ done, int ublk_cols,
+                                                             size_t ublk_stride) {
     const int node = order ? order[blockIdx.x] : blockIdx.x;
     if (nR > 0 && !include_boundary && (node % nR == 0 || node % nR == nR - 1)) return;
     if (done && done[node]) return;
     const int stride = n;
     if (nn) n = nn[node];
     const int kv = kl + ku;
-    const int WS = kv + NB;  // staged columns per block (relative to the block start)
+    const int WS = BLK ? ublk_cols : kv + NB;  // columns per block (relative to the block start)
     const int nI = (WS + NB - 1) / NB;
-    const T* ab = ab_all + (size_t)node * ldab * stride;
+    const T* ab = ab_all + (BLK ? (size_t)node * ublk_stride : (size_t)node * ldab * stride);
     const T* yv = y_all + (size_t)node * stride;
     T* xo = x_all + (size_t)node * stride;
     extern __shared__ double sh_raw[];  // 8-byte aligned
     T* sh = reinterpret_cast<T*>(sh_raw);
-    T* x = sh;                    // ring of kXRing: x[i] at i % kXRing
-    T* stg = sh + kXRing;         // 2 x [NB][WS]
-    T* part = stg + 2 * NB * WS;  // [NB][16] partial sums
+    T* x = sh;               // ring of kXRing: x[i] at i % kXRing
+    T* sd = sh + kXRing;     // [NB][NB] diagonal block of the current block row
+    T* part = sd + NB * NB;  // [NB][16] partial sums
     const int tid = threadIdx.x;
     const int ri = tid & 15, t = tid >> 4;
     const __amdgpu_buffer_rsrc_t rs =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(ab), (short)0, ldab * n * (int)sizeof(T), 0x00020000);
     const int nblk = (n + NB - 1) / NB;
+    (void)rs;
 
-    T va[kBsMaxI], vb[kBsMaxI];
-    T ya = T(0), yb = T(0);
     // this thread's entries of block blk: row jb+ri, columns jb + t + 16i
     // (and, for the 16 threads of the triangle, y of its row)
-    auto issue = [&](int blk, T (&v)[kBsMaxI], T& yr) {
+    auto issue = [&](int blk, T (&v)[NI], T& yr) {
         const int jb = NB * blk, r = jb + ri;
         if (tid < NB) yr = (r < n) ? yv[r] : T(0);
         const int hi = min(r + kv, n - 1);
-#pragma unroll
-        for (int i = 0; i < kBsMaxI; i++) {
-            const int co = t + NB * i;
-            const int col = jb + co;
-            const bool ok = (i < nI) & (co < WS) & (r < n) & (col >= r) & (col <= hi);
-            const unsigned off = (unsigned)(col * (ldab - 1) + kv + r) * (unsigned)sizeof(T);
-            v[i] = load_entry<T>(rs, ok ? off : kOutside);
+        if constexpr (BLK) {
+            // the block is contiguous and clean: entry (ri, t + 16i) at
+            // 256i + tid
+            const T* src = ab + (size_t)blk * NB * WS;
+            static_for<0, NI>([&](auto ic) {
+                constexpr int i = ic;
+                v[i] = (i < nI) ? src[NB * NB * i + tid] : T(0);
+            });
+        } else {
+            static_for<0, NI>([&](auto ic) {
+                constexpr int i = ic;
+                if (i < nI) {
+                    const int co = t + NB * i;
+                    const int col = jb + co;
+                    const bool ok = (co < WS) & (r < n) & (col >= r) & (col <= hi);
+                    const unsigned off = (unsigned)(col * (ldab - 1) + kv + r) * (unsigned)sizeof(T);
+                    v[i] = load_entry<T>(rs, ok ? off : kOutside);
+                } else {
+                    v[i] = T(0);
+                }
+            });
         }
     };
-    auto commit = [&](int buf, const T (&v)[kBsMaxI]) {
-        T* st = stg + buf * NB * WS;
-#pragma unroll
-        for (int i = 0; i < kBsMaxI; i++) {
-            const int co = t + NB * i;
-            if ((i < nI) & (co < WS)) st[ri * WS + co] = v[i];
-        }
-    };
-    // one block: partial sums over the band right of it, then the triangle
-    auto solve_block = [&](int blk, T yr) {
+    // one block: partial sums over the band right of it straight from the
+    // registers the block was loaded into, then the triangle
+    auto solve_block = [&](int blk, const T (&v)[NI], T yr) {
         const int jb = NB * blk;
-        const T* st = stg + (blk & 1) * NB * WS;
         {
-            T sacc = T(0);
             const int r = jb + ri;
-            if (r < n) {
-                const int hi = min(r + kv, n - 1) - jb;
-                for (int co = NB + t; co <= hi; co += NB) sacc += st[ri * WS + co] * x[(jb + co) & (kXRing - 1)];
-            }
+            sd[ri * NB + t] = v[0];  // column jb + t
+            const int hi = (r < n) ? min(r + kv, n - 1) - jb : -1;
+            T sacc = T(0);
+            static_for<1, NI>([&](auto ic) {
+                constexpr int i = ic;
+                const int co = t + NB * i;
+                const T xv = x[(jb + co) & (kXRing - 1)];
+                sacc = (co <= hi) ? fma(v[i], xv, sacc) : sacc;
+            });
             part[ri * 16 + t] = sacc;
         }
         __syncthreads();
@@ -1035,8 +1060,8 @@ __global__ __launch_bounds__(kBsThreads) void band_backsolve(const T* __restrict
                 for (int q = 0; q < 16; q++) sp += part[i * 16 + q];
                 b = yr - sp;
 #pragma unroll
-                for (int k = 0; k < NB; k++) urow[k] = st[i * WS + k];
-                rd = T(1) / st[i * WS + i];
+                for (int k = 0; k < NB; k++) urow[k] = sd[i * NB + k];
+                rd = T(1) / sd[i * NB + i];
             }
 #pragma unroll
             for (int k = NB - 1; k >= 0; k--) {
@@ -1050,31 +1075,29 @@ __global__ __launch_bounds__(kBsThreads) void band_backsolve(const T* __restrict
                 }
             }
         }
+        __syncthreads();
     };
 
-    // U blocks are requested two blocks ahead of their solve: while block blk
-    // is solved from LDS, block blk-1 waits in registers and block blk-2 is
-    // in flight
+    // U blocks are requested D-1 blocks ahead of their solve into D
+    // register sets used in turn
+    T v[D][NI];
+    T yq[D];
     const int top = nblk - 1;
-    T ycur = T(0);
-    issue(top, va, ycur);
-    commit(top & 1, va);
-    if (top >= 1) issue(top - 1, vb, yb);
-    __syncthreads();
-    for (int blk = top; blk >= 0; blk -= 2) {
-        // va is free, vb holds block blk-1
-        if (blk >= 2) issue(blk - 2, va, ya);
-        solve_block(blk, ycur);
-        ycur = yb;
-        if (blk >= 1) commit((blk - 1) & 1, vb);
-        __syncthreads();
-        if (blk < 1) break;
-        // vb is free, va holds block blk-2
-        if (blk >= 3) issue(blk - 3, vb, yb);
-        solve_block(blk - 1, ycur);
-        ycur = ya;
-        if (blk >= 2) commit((blk - 2) & 1, va);
-        __syncthreads();
+    static_for<0, D - 1>([&](auto sc) {
+        constexpr int sset = sc;
+        yq[sset] = T(0);
+        if (top - sset >= 0) issue(top - sset, v[sset], yq[sset]);
+    });
+    yq[D - 1] = T(0);
+    for (int blk = top; blk >= 0; blk -= D) {
+        static_for<0, D>([&](auto kc) {
+            constexpr int k = kc;
+            constexpr int nxt = (k + D - 1) % D;  // the set freed by the previous solve
+            const int b = blk - k;
+            if (b < 0) return;
+            if (b - (D - 1) >= 0) issue(b - (D - 1), v[nxt], yq[nxt]);
+            solve_block(b, v[k], yq[k]);
+        });
     }
 }
 
@@ -1233,28 +1256,58 @@ int launch_forward32(const float* ab, const int64_t* ipiv, int n, int kl, int ku
                                    stream, done);
 }
 
+template <typename T, bool BLK, int NI, int D>
+int launch_backsolve_v(const T* ab, const T* y, T* x, int n, int kl, int ku, int ldab, int nodes, int nR,
+                       int include_boundary, hipStream_t stream, const int* nn, const int* order, const int* done,
+                       int ublk_cols, size_t ublk_stride, size_t lds) {
+    hipFuncSetAttribute((const void*)band_backsolve<T, BLK, NI, D>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                        (int)lds);
+    hipLaunchKernelGGL((band_backsolve<T, BLK, NI, D>), dim3(nodes), dim3(kBsThreads), lds, stream, ab, y, x, n, kl,
+                       ku, ldab, nR, include_boundary, nn, order, done, ublk_cols, ublk_stride);
+    return hipGetLastError() == hipSuccess ? VERB_OK : VERB_ERR_HIP;
+}
+
+// Register sets per thread: NI 16-column groups of a block (the smallest
+// instantiation that covers kl+ku+16 columns) times D blocks in flight.
+template <typename T, bool BLK>
+int launch_backsolve_b(const T* ab, const T* y, T* x, int n, int kl, int ku, int ldab, int nodes, int nR,
+                       int include_boundary, hipStream_t stream, const int* nn, const int* order, const int* done,
+                       int ublk_cols, size_t ublk_stride, size_t lds) {
+    const int cols = BLK ? ublk_cols : kl + ku + NB;
+    if (cols <= NB * 8)
+        return launch_backsolve_v<T, BLK, 8, 6>(ab, y, x, n, kl, ku, ldab, nodes, nR, include_boundary, stream, nn,
+                                                order, done, ublk_cols, ublk_stride, lds);
+    if (cols <= NB * 14)
+        return launch_backsolve_v<T, BLK, 14, 4>(ab, y, x, n, kl, ku, ldab, nodes, nR, include_boundary, stream, nn,
+                                                 order, done, ublk_cols, ublk_stride, lds);
+    return launch_backsolve_v<T, BLK, kBsMaxI, 3>(ab, y, x, n, kl, ku, ldab, nodes, nR, include_boundary, stream, nn,
+                                                  order, done, ublk_cols, ublk_stride, lds);
+}
+
 template <typename T>
 int launch_backsolve_t(const T* ab, const T* y, T* x, int n, int kl, int ku, int ldab, int nodes, int nR,
-                       int include_boundary, hipStream_t stream, const int* nn, const int* order, const int* done) {
+                       int include_boundary, hipStream_t stream, const int* nn, const int* order, const int* done,
+                       int ublk_cols = 0, size_t ublk_stride = 0) {
     if (kl + ku + NB > NB * kBsMaxI || kl + ku + 2 * NB > kXRing) return VERB_ERR_ARG;
-    const int WS = kl + ku + NB;
-    const size_t lds = sizeof(T) * ((size_t)kXRing + 2 * NB * WS + NB * 16);
+    if (ublk_cols && (ublk_cols > NB * kBsMaxI || ublk_cols < kl + ku + NB)) return VERB_ERR_ARG;
+    const size_t lds = sizeof(T) * ((size_t)kXRing + NB * NB + NB * 16);
     int dev = 0, max_lds = 0;
     hipGetDevice(&dev);
     hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
     if (lds + 256 > (size_t)max_lds) return VERB_ERR_ARG;
-    hipFuncSetAttribute((const void*)band_backsolve<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(band_backsolve<T>, dim3(nodes), dim3(kBsThreads), lds, stream, ab, y, x, n, kl, ku, ldab, nR,
-                       include_boundary, nn, order, done);
-    return hipGetLastError() == hipSuccess ? VERB_OK : VERB_ERR_HIP;
+    if (ublk_cols)
+        return launch_backsolve_b<T, true>(ab, y, x, n, kl, ku, ldab, nodes, nR, include_boundary, stream, nn, order,
+                                           done, ublk_cols, ublk_stride, lds);
+    return launch_backsolve_b<T, false>(ab, y, x, n, kl, ku, ldab, nodes, nR, include_boundary, stream, nn, order,
+                                        done, 0, 0, lds);
 }
 
 int launch_backsolve(const double* ab, const double* y, const int* uext, double* x, int n, int kl, int ku, int ldab,
                      int nodes, int nR, int include_boundary, hipStream_t stream, const int* nn, const int* order,
-                     const int* done) {
+                     const int* done, int ublk_cols, size_t ublk_stride) {
     (void)uext;  // U rows are stored up to kl+ku past the diagonal, so extents are not needed
     return launch_backsolve_t<double>(ab, y, x, n, kl, ku, ldab, nodes, nR, include_boundary, stream, nn, order,
-                                      done);
+                                      done, ublk_cols, ublk_stride);
 }
 
 int launch_backsolve32(const float* ab, const float* y, float* x, int n, int kl, int ku, int ldab, int nodes, int nR,

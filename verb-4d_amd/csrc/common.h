@@ -52,7 +52,8 @@ int lu_variant_count();
 int lu_variant_shape(int i, int kl, int* rb, int* cb);
 int launch_backsolve(const double* ab, const double* y, const int* uext, double* x, int n, int kl, int ku, int ldab,
                      int nodes, int nR, int include_boundary, hipStream_t stream, const int* nn = nullptr,
-                     const int* order = nullptr, const int* done = nullptr);
+                     const int* order = nullptr, const int* done = nullptr, int ublk_cols = 0,
+                     size_t ublk_stride = 0);
 // y = L^-1 P (B0*rhs + C0) (B0, C0 nullable: y = L^-1 P rhs) with LAPACK
 // band factors (dgbtf2-order multipliers); kl <= 112.
 int launch_forward(const double* ab, const int64_t* ipiv, int n, int kl, int ku, int ldab, const double* b0,

@@ -530,6 +530,9 @@ int launch_local(const verb_grid& g, double* psd, const LocalWork& w, bool facto
         la.include_boundary = g.include_boundary;
         la.nn = nns; la.order = order;
         la.queue = w.queue; la.qnodes = nodes;
+        // U goes out in 16-row blocks (the factors are not kept), which the
+        // back-substitution streams
+        if (!ublk_geometry(n, kl, ku, &la.ublk_cols, &la.ublk_stride)) la.ublk_cols = 0;
         int rc = launch_band_lu(la, nodes, stream, w.lu_variant);
         if (rc) return rc;
         if (marks) {
@@ -537,7 +540,7 @@ int launch_local(const verb_grid& g, double* psd, const LocalWork& w, bool facto
             hipEventRecord(marks[4], stream);
         }
         rc = launch_backsolve(w.ab, w.y, w.uext, w.rhs, n, kl, ku, ldab, nodes, g.nR, g.include_boundary, stream, nns,
-                              order);
+                              order, nullptr, la.ublk_cols, la.ublk_stride);
         if (rc) return rc;
         hipLaunchKernelGGL(scatter_psd, agrid, dim3(kAsmThreads), 0, stream, g, w.rhs, psd, k0s);
         if (marks) hipEventRecord(marks[5], stream);
@@ -655,10 +658,11 @@ int launch_local_gmres(const verb_grid& g, double* psd, const LocalWork& w, cons
     la.include_boundary = g.include_boundary;
     la.nn = nullptr; la.order = nullptr; la.done = gw.conv;
     la.queue = w.queue; la.qnodes = nodes;
+    if (!ublk_geometry(n, kl, ku, &la.ublk_cols, &la.ublk_stride)) la.ublk_cols = 0;
     rc = launch_band_lu(la, nodes, stream, w.lu_variant);
     if (rc) return rc;
     rc = launch_backsolve(w.ab, w.y, w.uext, gw.x, n, kl, ku, ldab, nodes, g.nR, g.include_boundary, stream, nullptr,
-                          nullptr, gw.conv);
+                          nullptr, gw.conv, la.ublk_cols, la.ublk_stride);
     if (rc) return rc;
     hipLaunchKernelGGL(scatter_psd, agrid, dim3(kAsmThreads), 0, stream, g, gw.x, psd, (const int*)nullptr);
     if (marks) hipEventRecord(marks[5], stream);
