@@ -12,26 +12,28 @@
 //
 // Blocking. The matrix is energy-fast (kl = ku = nV+1). Panels are NB = 16
 // columns wide. At panel p the live window is RB row blocks x CB column
-// blocks of 16x16 FP64 tiles (RB*16 >= kl+16, CB*16 >= kl+ku+16); row block
-// b lives in wavefront b mod RB and column block c in register slot c mod CB
-// of every wavefront, so the window slides by one block per panel without
-// moving data: a finished row block's wavefront reloads the row block
-// entering at the bottom, every wavefront reloads the slot of the finished
-// column block with the column block entering on the right, both straight
-// from the assembled 9-diagonal operator (entries that enter the window have
-// not yet been touched by any elimination step).
+// blocks of 16x16 tiles in MFMA accumulator layout (RB*16 >= kl+16,
+// CB*16 >= kl+ku+16); row block b lives in wavefront b mod RB, and slot s of
+// every wavefront holds column block p+s (the registers rotate by one slot per
+// panel, so every tile offset is a compile-time constant). Tiles that enter
+// the window come straight from the assembled 9-diagonal operator (entries
+// that enter the window have not yet been touched by any elimination step).
 //
-// Per panel:
-//   1. every wavefront copies its tile of the panel column block to LDS;
-//   2. wavefront 0 factors the panel (118 x 16 at kl = 102) in LDS with
-//      full-row interchanges and eliminates the right-hand side window;
-//   3. the panel's 16 interchanges are applied to the rest of the window as
-//      one net permutation through LDS, and U12 = L11^-1 P A12 is formed by
-//      one lane per column;
-//   4. A22 -= L21 U12 with v_mfma_f64_16x16x4f64 on the register tiles,
-//      skipping column blocks beyond the running fill extent ju; the panel
-//      wavefront meanwhile streams the U rows (and, for the cached path, the
-//      multipliers re-ordered to the dgbtf2 convention) to HBM.
+// Look-ahead schedule, four barriers per panel:
+//   stage p: one wavefront (the one that held the panel rows of p-1) factors
+//      panel p in registers with full-row interchanges and eliminates the
+//      right-hand-side window, while the others finish the trailing update
+//      of panel p-1 on MFMA, stream its U rows to HBM and fetch the factoring
+//      wavefront's new row block into an LDS stash;
+//   3a: the rows that the panel's interchanges move are staged to LDS, and
+//      the factoring wavefront forms L11^-1;
+//   3b/3c: U12 = L11^-1 P A12 on MFMA; moved rows below the panel take their
+//      new contents;
+//   4a: column block p+1 is updated first and written to the other panel
+//      buffer: it is panel p+1.
+// U goes out in the LAPACK band layout (and, for the cached path, with the
+// multipliers re-ordered to the dgbtf2 convention) or, when the factors are
+// not kept, in contiguous 16-row blocks that the back-substitution streams.
 #include <type_traits>
 #include <utility>
 
