@@ -1280,7 +1280,7 @@ int launch_backsolve_b(const T* ab, const T* y, T* x, int n, int kl, int ku, int
         return launch_backsolve_v<T, BLK, 8, 6>(ab, y, x, n, kl, ku, ldab, nodes, nR, include_boundary, stream, nn,
                                                 order, done, ublk_cols, ublk_stride, lds);
     if (cols <= NB * 14)
-        return launch_backsolve_v<T, BLK, 14, 4>(ab, y, x, n, kl, ku, ldab, nodes, nR, include_boundary, stream, nn,
+        return launch_backsolve_v<T, BLK, 14, 3>(ab, y, x, n, kl, ku, ldab, nodes, nR, include_boundary, stream, nn,
                                                  order, done, ublk_cols, ublk_stride, lds);
     return launch_backsolve_v<T, BLK, kBsMaxI, 3>(ab, y, x, n, kl, ku, ldab, nodes, nR, include_boundary, stream, nn,
                                                   order, done, ublk_cols, ublk_stride, lds);
