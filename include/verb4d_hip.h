@@ -177,6 +177,29 @@ int verb_step_uneven(verb_ctx* ctx, const verb_grid* g, double* psd, const int* 
 int verb_step(verb_ctx* ctx, const verb_grid* g, double* psd, int run_convection, int run_radial, int run_local,
               double* lu_cache, int64_t* ipiv_cache, int recompute_lu);
 
+/* The step as the reference's time loop runs it around the three operators
+ * (src/VERB4D_Solver.cpp:539-1282), on the device-resident grid:
+ *   positive_psd != 0: PSD.max_of(1e-21) before the step and after each
+ *     operator (:621, 980, 1072, 1218-1221), max_of being
+ *     (v < 1e-21) ? 1e-21 : v (src/Matrix.cpp:5948-5953);
+ *   vl_from_convection (device [P][R][K] buffer, or NULL): when convection
+ *     runs and Vl_type is ConstantValue, it takes PSD[:, :, 0, :] after
+ *     convection and is the V-lower boundary of this step's local diffusion
+ *     (:965-966, Vl_BC = PSD.ySlice(0)); the caller keeps it as g->Vl for
+ *     later steps as the reference keeps Vl_BC until its next update;
+ *   negative_points (host pointer, or NULL): with positive_psd == 0, the
+ *     number of PSD values below 0 after the step (:1223-1238); reading it
+ *     synchronises the stream.
+ * opt == NULL is verb_step. */
+typedef struct {
+    int positive_psd;
+    double* vl_from_convection;
+    long long* negative_points;
+} verb_step_options;
+int verb_step_ex(verb_ctx* ctx, const verb_grid* g, double* psd, int run_convection, int run_radial,
+                 int run_local, double* lu_cache, int64_t* ipiv_cache, int recompute_lu,
+                 const verb_step_options* opt);
+
 /* Pulls the device-side status word of the last batched launches (Courant
  * violation, zero pivot) and synchronises the stream. */
 int verb_check_status(verb_ctx* ctx);

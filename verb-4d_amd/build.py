@@ -24,6 +24,7 @@ SOURCES = [
     ("local.hip", ["-ffp-contract=off"]),
     ("band_lu.hip", []),
     ("gmres.hip", []),
+    ("driver.hip", []),
     ("api.hip", []),
 ]
 

@@ -702,6 +702,44 @@ int verb_step(verb_ctx* c, const verb_grid* g, double* psd, int run_convection, 
     return rc;
 }
 
+constexpr int kSlotDriver = 95;  // negative-point counter
+
+int verb_step_ex(verb_ctx* c, const verb_grid* g, double* psd, int run_convection, int run_radial, int run_local,
+                 double* lu_cache, int64_t* ipiv_cache, int recompute_lu, const verb_step_options* opt) {
+    if (!opt) return verb_step(c, g, psd, run_convection, run_radial, run_local, lu_cache, ipiv_cache, recompute_lu);
+    if (!c || !g || !psd) return VERB_ERR_ARG;
+    const size_t n = (size_t)g->nP * g->nR * g->nV * g->nK;
+    constexpr double kFloor = 1e-21;
+    int rc = VERB_OK;
+    if (opt->positive_psd) rc = verb::launch_clamp_floor(psd, n, kFloor, c->stream);
+    verb_grid gl = *g;
+    if (!rc && run_convection && (3 < g->nP || 3 < g->nR)) {
+        rc = verb_convection_batched(c, g, psd);
+        if (!rc && opt->vl_from_convection && g->Vl_type == VERB_BC_CONSTANT_VALUE) {
+            rc = verb::launch_vl_from_psd(*g, psd, opt->vl_from_convection, c->stream);
+            gl.Vl = opt->vl_from_convection;
+        }
+    }
+    if (!rc && opt->positive_psd) rc = verb::launch_clamp_floor(psd, n, kFloor, c->stream);
+    if (!rc && run_radial && g->nR >= 3) rc = verb_radial_diffusion_batched(c, &gl, psd);
+    if (!rc && opt->positive_psd) rc = verb::launch_clamp_floor(psd, n, kFloor, c->stream);
+    if (!rc && run_local && g->nV >= 3 && g->nK >= 3)
+        rc = verb_local_diffusion_batched(c, &gl, psd, lu_cache, ipiv_cache, recompute_lu);
+    if (rc) return rc;
+    if (opt->positive_psd) return verb::launch_clamp_floor(psd, n, kFloor, c->stream);
+    if (opt->negative_points) {
+        auto* cnt = (unsigned long long*)scratch(c, kSlotDriver, sizeof(unsigned long long));
+        if (!cnt) return set_err(c, VERB_ERR_NOMEM, "device allocation failed");
+        rc = verb::launch_count_negative(psd, n, cnt, c->stream);
+        if (rc) return set_err(c, rc, "negative count launch failed");
+        unsigned long long h = 0;
+        HIPCHK(c, hipMemcpyAsync(&h, cnt, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        *opt->negative_points = (long long)h;
+    }
+    return VERB_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Per-call drop-in entries
 // ---------------------------------------------------------------------------

@@ -124,4 +124,9 @@ int launch_dgbtrs(int n, int kl, int ku, const double* ab, int ldab, const int64
 int launch_tridag(const double* a, const double* b, const double* c, const double* r, double* u, int n,
                   int lines, hipStream_t stream);
 
+// Time-loop pieces around the operators (driver.hip).
+int launch_clamp_floor(double* psd, size_t n, double floor, hipStream_t stream);
+int launch_count_negative(const double* psd, size_t n, unsigned long long* count, hipStream_t stream);
+int launch_vl_from_psd(const verb_grid& g, const double* psd, double* vl, hipStream_t stream);
+
 }  // namespace verb

@@ -45,6 +45,13 @@ class VerbError(RuntimeError):
         self.status = status
 
 
+class StepOptions(ctypes.Structure):
+    """verb_step_options: the time loop's clamp, V-lower boundary from
+    convection and negative-point count around the step."""
+    _fields_ = [("positive_psd", ctypes.c_int), ("vl_from_convection", ctypes.c_void_p),
+                ("negative_points", ctypes.POINTER(ctypes.c_longlong))]
+
+
 class GmresParams(ctypes.Structure):
     """ctypes mirror of ``verb_gmres_params`` (include/verb4d_hip.h)."""
 
@@ -173,6 +180,9 @@ _SIGNATURES = [
     ("verb_get_tuning", ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),
     ("verb_autotune", ctypes.c_int, [_P, ctypes.POINTER(VerbGrid), ctypes.c_void_p, ctypes.POINTER(Tuning)]),
     ("verb_step_graph", ctypes.c_int, [_P, ctypes.POINTER(VerbGrid), ctypes.c_void_p, ctypes.c_void_p]),
+    ("verb_step_ex", ctypes.c_int,
+     [_P, ctypes.POINTER(VerbGrid), ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+      ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(StepOptions)]),
 ]
 
 

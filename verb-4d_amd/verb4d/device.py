@@ -8,7 +8,7 @@ import ctypes
 
 import numpy as np
 
-from . import Context, GmresParams, GmresStats, MixedParams, MixedStats, Tuning, VerbGrid, lib
+from . import Context, GmresParams, GmresStats, MixedParams, MixedStats, StepOptions, Tuning, VerbGrid, lib
 
 
 class DeviceGrid:
@@ -71,6 +71,21 @@ class DeviceGrid:
         rc = lib().verb_step(self.ctx.handle, ctypes.byref(self.grid), self.psd.data_ptr(), int(run_convection),
                              int(run_radial), int(run_local), lu, piv, 1 if recompute_lu else 0)
         self.ctx.check(rc)
+
+    def step_ex(self, positive_psd=False, vl_from_convection=None, count_negative=False, run_convection=True,
+                run_radial=True, run_local=True):
+        """verb_step_ex: the step with the time loop's positivity clamps,
+        the V-lower boundary taken from the PSD after convection (into the
+        given device tensor [P][R][K]) and the count of negative points.
+        Returns the count (or None)."""
+        cnt = ctypes.c_longlong(-1)
+        opt = StepOptions(1 if positive_psd else 0,
+                          vl_from_convection.data_ptr() if vl_from_convection is not None else None,
+                          ctypes.pointer(cnt) if count_negative else None)
+        rc = lib().verb_step_ex(self.ctx.handle, ctypes.byref(self.grid), self.psd.data_ptr(), int(run_convection),
+                                int(run_radial), int(run_local), None, None, 1, ctypes.byref(opt))
+        self.ctx.check(rc)
+        return cnt.value if count_negative else None
 
     def convection(self):
         self.ctx.check(lib().verb_convection_batched(self.ctx.handle, ctypes.byref(self.grid), self.psd.data_ptr()))
