@@ -173,6 +173,80 @@ __global__ __launch_bounds__(kAsmThreads) void local_assemble(const verb_grid g,
     }
 }
 
+// The same assembly when no LAPACK band is written, by 16x16 tiles of
+// (ix, iy): each point is computed by the lane whose [V][K] reads are
+// consecutive in iy and written by the lane whose energy-fast outputs are
+// consecutive in ix, the tile being transposed through LDS.
+constexpr int kAsmTile = 16;
+
+__global__ __launch_bounds__(kAsmTile* kAsmTile) void local_assemble_tiled(
+    const verb_grid g, const double* __restrict__ psd, double* __restrict__ b0, double* __restrict__ c0,
+    double* __restrict__ rhs, double* __restrict__ diag, double dt_sub, int premul, int* status,
+    const int* __restrict__ k0s) {
+    const int node = blockIdx.y;
+    const int iP = node / g.nR, iR = node % g.nR;
+    if (node_skipped(g, iR)) return;
+    const int k0 = k0s ? k0s[node] : 0;
+    const int nx = g.nV, ny = g.nK - k0, n = nx * ny;
+    const int nfull = g.nV * g.nK;
+    const int tiles_y = (g.nK + kAsmTile - 1) / kAsmTile;
+    const int ix0 = (blockIdx.x / tiles_y) * kAsmTile, iy0 = (blockIdx.x % tiles_y) * kAsmTile;
+    if (ix0 >= nx || iy0 >= ny) return;
+    const int tx = threadIdx.x % kAsmTile, ty = threadIdx.x / kAsmTile;
+    __shared__ double out[12][kAsmTile][kAsmTile + 1];
+    const size_t nn = (size_t)node * nfull;
+    {
+        const int ix = ix0 + ty, iy = iy0 + tx;
+        if (ix < nx && iy < ny) {
+            const NodeArgs a = node_args(g, iP, iR, dt_sub, k0);
+            double A[9], B0, C0;
+            if (!assemble_row(a, ix, iy, A, B0, C0)) raise_error(status, VERB_ERR_BC);
+            const size_t pp = nn + (size_t)ix * g.nK + k0 + iy;
+            double v;
+            if (premul) {
+                // first sub-step's right-hand side: rhs = rhs*B0 + C0 (MatrixSolver.cpp:318-319)
+                v = psd[pp] * B0;
+                v += C0;
+            } else {
+                v = psd[pp];
+            }
+#pragma unroll
+            for (int sl = 0; sl < 9; sl++) out[sl][ty][tx] = A[sl];
+            out[9][ty][tx] = B0;
+            out[10][ty][tx] = C0;
+            out[11][ty][tx] = v;
+        }
+    }
+    __syncthreads();
+    const int ix = ix0 + tx, iy = iy0 + ty;
+    if (ix < nx && iy < ny) {
+        const int in = iy * nx + ix;
+        b0[nn + in] = out[9][tx][ty];
+        c0[nn + in] = out[10][tx][ty];
+        rhs[nn + in] = out[11][tx][ty];
+        if (diag) {
+            double* dn = diag + (size_t)node * 9 * nfull;
+#pragma unroll
+            for (int sl = 0; sl < 9; sl++) dn[(size_t)sl * n + in] = out[sl][tx][ty];
+        }
+    }
+}
+
+void launch_assemble(const verb_grid& g, const double* psd, double* ab, double* b0, double* c0, double* rhs,
+                     double* diag, double dt_sub, int write_band, int premul, int* status, const int* k0s,
+                     hipStream_t stream) {
+    const int nodes = g.nP * g.nR;
+    if (write_band) {
+        const int n = g.nV * g.nK;
+        hipLaunchKernelGGL(local_assemble, dim3((n + kAsmThreads - 1) / kAsmThreads, nodes), dim3(kAsmThreads), 0,
+                           stream, g, psd, ab, b0, c0, rhs, diag, dt_sub, write_band, premul, status, k0s);
+    } else {
+        const int tiles = ((g.nV + kAsmTile - 1) / kAsmTile) * ((g.nK + kAsmTile - 1) / kAsmTile);
+        hipLaunchKernelGGL(local_assemble_tiled, dim3(tiles, nodes), dim3(kAsmTile * kAsmTile), 0, stream, g, psd,
+                           b0, c0, rhs, diag, dt_sub, premul, status, k0s);
+    }
+}
+
 __global__ void fill_zero(double* __restrict__ p, size_t count) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -518,8 +592,7 @@ int launch_local(const verb_grid& g, double* psd, const LocalWork& w, bool facto
     if (window && factor && num_substeps == 1 && !cached) {
         // plain path: factor with the forward elimination fused, then U x = y
         if (marks) hipEventRecord(marks[0], stream);
-        hipLaunchKernelGGL(local_assemble, agrid, dim3(kAsmThreads), 0, stream, g, psd, w.ab, w.b0, w.c0, w.rhs,
-                           w.diag, dt_sub, 0, 1, status, k0s);
+        launch_assemble(g, psd, w.ab, w.b0, w.c0, w.rhs, w.diag, dt_sub, 0, 1, status, k0s, stream);
         if (marks) {
             hipEventRecord(marks[1], stream);
             hipEventRecord(marks[2], stream);
@@ -552,9 +625,8 @@ int launch_local(const verb_grid& g, double* psd, const LocalWork& w, bool facto
         hipLaunchKernelGGL(fill_zero, dim3(4096), dim3(256), 0, stream, w.ab, count);
     }
     if (sizes && !window) return VERB_ERR_ARG;  // uneven extents need the register-window LU
-    hipLaunchKernelGGL(local_assemble, agrid, dim3(kAsmThreads), 0, stream, g, psd, w.ab, w.b0, w.c0, w.rhs,
-                       (window && factor) ? w.diag : (double*)nullptr, dt_sub, (factor && !window) ? 1 : 0, 0,
-                       status, k0s);
+    launch_assemble(g, psd, w.ab, w.b0, w.c0, w.rhs, (window && factor) ? w.diag : (double*)nullptr, dt_sub,
+                    (factor && !window) ? 1 : 0, 0, status, k0s, stream);
     if (marks) {
         hipEventRecord(marks[1], stream);
         hipEventRecord(marks[2], stream);
@@ -633,8 +705,7 @@ int launch_local_gmres(const verb_grid& g, double* psd, const LocalWork& w, cons
     const double dt_sub = g.dt / num_substeps;
     dim3 agrid((n + kAsmThreads - 1) / kAsmThreads, nodes);
     if (marks) hipEventRecord(marks[0], stream);
-    hipLaunchKernelGGL(local_assemble, agrid, dim3(kAsmThreads), 0, stream, g, psd, w.ab, w.b0, w.c0, w.rhs, w.diag,
-                       dt_sub, 0, 1, status, (const int*)nullptr);
+    launch_assemble(g, psd, w.ab, w.b0, w.c0, w.rhs, w.diag, dt_sub, 0, 1, status, nullptr, stream);
     hipLaunchKernelGGL(gather_psd, agrid, dim3(kAsmThreads), 0, stream, g, psd, gw.x);
     if (marks) {
         hipEventRecord(marks[1], stream);
@@ -681,8 +752,7 @@ int launch_local_mixed(const verb_grid& g, double* psd, const LocalWork& w, cons
     const double dt_sub = g.dt;
     dim3 agrid((n + kAsmThreads - 1) / kAsmThreads, nodes);
     if (marks) hipEventRecord(marks[0], stream);
-    hipLaunchKernelGGL(local_assemble, agrid, dim3(kAsmThreads), 0, stream, g, psd, w.ab, w.b0, w.c0, w.rhs, w.diag,
-                       dt_sub, 0, 1, status, (const int*)nullptr);
+    launch_assemble(g, psd, w.ab, w.b0, w.c0, w.rhs, w.diag, dt_sub, 0, 1, status, nullptr, stream);
     hipLaunchKernelGGL(mixed_init, dim3(nodes), dim3(kMxThreads), 0, stream, g, w.diag, w.rhs, mw.d32, mw.x,
                        mw.bmax, mw.done, mw.sweeps);
     if (marks) {
