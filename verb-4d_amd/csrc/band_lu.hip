@@ -372,20 +372,27 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
             T x[NQ][NB];
             int pos[NQ];
             T rh[NQ];
-            bool bad = false;
+            // (loads from a clamped row, so that they need no branch; a
+            // non-finite entry turns the sum of the entries times zero into
+            // NaN)
+            T chk[2] = {T(0), T(0)};
             static_for<0, NQ>([&](auto qc) {
                 constexpr int q = qc;
                 const int row = l + 64 * q;
-                const bool h = row < PR;
+                const bool h = 64 * (q + 1) <= PR || row < PR;
+                const int rc = h ? row : PR - 1;
                 static_for<0, NB>([&](auto kc) {
                     constexpr int k = kc;
-                    x[q][k] = h ? pn[k * PR + row] : T(0);
-                    bad |= !isfinite(x[q][k]);
+                    const T v = pn[k * PR + rc];
+                    x[q][k] = h ? v : T(0);
+                    chk[k & 1] = fma(x[q][k], T(0), chk[k & 1]);
                 });
                 pos[q] = h ? row : kNone;
-                rh[q] = h ? S.rw[(j0 + row) % PR] : T(0);
-                bad |= !isfinite(rh[q]);
+                const T r = S.rw[(j0 + rc) % PR];
+                rh[q] = h ? r : T(0);
+                chk[0] = fma(rh[q], T(0), chk[0]);
             });
+            const bool bad = !isfinite(chk[0] + chk[1]);
             // Columns are taken by an unrolled sequence with the column index
             // known at compile time (only entries right of the step are
             // touched) while the panel holds only finite values, the largest
@@ -420,7 +427,9 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
                     int cnt = 0;
                     static_for<0, NQ>([&](auto qc) {
                         constexpr int q = qc;
-                        b[q] = __ballot(cd[q] & (kk[q] == m));
+                        // (non-candidates have key 0, and a zero maximum
+                        // matches every lane)
+                        b[q] = __ballot(kk[q] == m);
                         cnt += __popcll(b[q]);
                     });
                     if (cnt != 1) return;
@@ -945,6 +954,10 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
 #pragma unroll
                 for (int r = 0; r < 4; r++) pnx[jj * PR + NB * (RB - 1) + P::crow(g, r)] = acc[0][r];
             }
+            // This wavefront factors the next panel: its global stores are
+            // waited for here, where it has time to spare, rather than at the
+            // start of the factorisation.
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         }
         __syncthreads();
         ju_prev = ju;
