@@ -357,6 +357,11 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
         //    end every row is stored at its final position (the result of
         //    full-row interchanges).
         if (fac && p < npan) {
+            // The factorisation is the critical path of the stage and runs
+            // on one wavefront; it shares its SIMD with a wavefront of the
+            // trailing update, which has time to spare: issue priority goes
+            // to the factorisation.
+            __builtin_amdgcn_s_setprio(3);
             constexpr int NQ = (PR + 63) / 64;
             static_assert(NQ >= 1 && NQ <= 3, "window rows per lane");
             const int l = lane_fresh();
@@ -646,6 +651,7 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
                 S.info = info;
                 S.lfin = !nonfin;
             }
+            __builtin_amdgcn_s_setprio(0);
         }
         else if (!fac && p > 0) {
             // 4b. trailing update of panel q = p-1 on the remaining column
@@ -1062,6 +1068,9 @@ __global__ __launch_bounds__(kBsThreads) void band_backsolve(const T* __restrict
         }
         __syncthreads();
         if (tid < 64) {
+            // (the triangle is the serial part of the block: issue priority
+            // over the wavefronts of the other workgroups on the CU)
+            __builtin_amdgcn_s_setprio(3);
             const int i = tid & 15;
             const int r = jb + i;
             const bool live = (tid < NB) & (r < n);
@@ -1089,6 +1098,7 @@ __global__ __launch_bounds__(kBsThreads) void band_backsolve(const T* __restrict
                     b = (i < k) ? fma(-urow[k], xk, b) : b;
                 }
             }
+            __builtin_amdgcn_s_setprio(0);
         }
         __syncthreads();
     };
