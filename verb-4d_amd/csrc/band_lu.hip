@@ -760,12 +760,21 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
                 // the L11 entries of columns [I0, I1) are requested together
                 // (one wait per group instead of one per column)
                 T lc[NB * NB];
+                // (16-byte reads at immediate offsets from the panel base:
+                // column i starts on a 16-byte boundary)
+                constexpr int E = 16 / sizeof(T);
+                using V16 = T __attribute__((ext_vector_type(E)));
                 auto fetch = [&](auto i0c, auto i1c) {
                     static_for<decltype(i0c)::value, decltype(i1c)::value>([&](auto ic) {
                         constexpr int i = ic;
-                        static_for<i + 1, NB>([&](auto cc) {
-                            constexpr int c = cc;
-                            lc[i * NB + c] = pn[i * PR + c];
+                        static_for<(i + 1) / E, NB / E>([&](auto vc) {
+                            constexpr int cv = E * vc;
+                            const V16 t = *reinterpret_cast<const V16*>(pn + i * PR + cv);
+                            static_for<0, E>([&](auto ec) {
+                                constexpr int e = ec;
+                                constexpr int c = cv + e;
+                                if constexpr (c > i) lc[i * NB + c] = t[e];
+                            });
                         });
                     });
                 };
