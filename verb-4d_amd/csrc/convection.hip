@@ -156,8 +156,11 @@ __device__ inline double limited_face(const double UW[10], double ou, double oc,
 
 struct Tile {
     int nP, nR, TQ, npt;
+    int tq_shift;   // log2(TQ) (TQ divides the 256 threads, so it is a power of two)
+    unsigned mR;    // ceil(2^32 / nR): pr / nR = umulhi(pr, mR) for pr * nR < 2^32
     long NQ;
     long q0;
+    const double* bc;  // per-line boundary data in LDS: [dir][line][lbc, ubc, sl, su]
 };
 
 // Geometry of the line through point (p, r, tq) for sweep dir (0 = MLT
@@ -177,10 +180,10 @@ __device__ inline LineGeom line_of(const Tile& T, int dir, int p, int r, int tq)
 }
 
 __device__ inline void decompose(const Tile& T, int t, int& p, int& r, int& tq) {
-    tq = t % T.TQ;
-    const int pr = t / T.TQ;
-    r = pr % T.nR;
-    p = pr / T.nR;
+    tq = t & (T.TQ - 1);
+    const int pr = t >> T.tq_shift;
+    p = (T.nR == 1) ? pr : (int)__umulhi((unsigned)pr, T.mR);
+    r = pr - p * T.nR;
 }
 
 // Boundary data of a line (values, and the ConstantDerivative steps).
@@ -224,6 +227,20 @@ __device__ inline LineBC line_bc(const verb_grid& g, const Tile& T, int dir, int
     return bc;
 }
 
+// Boundary data of line `line` of sweep dir, as line_bc computed it at the
+// start of the call (the data do not change between sub-steps).
+__device__ inline LineBC line_bc_lds(const verb_grid& g, const Tile& T, int dir, int line) {
+    LineBC bc;
+    bc.lt = dir == 0 ? g.Pl_type : g.Rl_type;
+    bc.ut = dir == 0 ? g.Pu_type : g.Ru_type;
+    const double* e = T.bc + 4 * (dir == 0 ? line : T.nR * T.TQ + line);
+    bc.lbc = e[0];
+    bc.ubc = e[1];
+    bc.sl = e[2];
+    bc.su = e[3];
+    return bc;
+}
+
 __device__ inline double face_value_or_zero(const double* FC, const double* F, int idx, bool has) {
     return has ? FC[idx] * F[idx] : 0.0 * 0.0;
 }
@@ -243,7 +260,7 @@ __device__ void quickest_step(const verb_grid& g, const Tile& T, int dir, int k,
         if (dir == 0 && r == T.nR - 1) continue;
         const LineGeom L = line_of(T, dir, p, r, tq);
         if (k >= ninner[L.line]) continue;
-        const LineBC bc = line_bc(g, T, dir, p, r, tq, Gc);
+        const LineBC bc = line_bc_lds(g, T, dir, L.line);
         const bool has = (L.ix > 0) || (bc.lt == VERB_BC_PERIODIC);
         if (!has) continue;
         double W[10];
@@ -260,7 +277,7 @@ __device__ void quickest_step(const verb_grid& g, const Tile& T, int dir, int k,
         LineGeom L = line_of(T, dir, p, r, tq);
         if (run && k >= ninner[L.line]) run = false;
         if (!run) { Y[t] = X[t]; continue; }
-        const LineBC bc = line_bc(g, T, dir, p, r, tq, Gc);
+        const LineBC bc = line_bc_lds(g, T, dir, L.line);
         const int n = L.n;
         int i = L.ix;
         double fixed = 0.0;
@@ -288,7 +305,7 @@ __device__ void quickest_step(const verb_grid& g, const Tile& T, int dir, int k,
         if (dir == 0 && r == T.nR - 1) continue;
         const LineGeom L = line_of(T, dir, p, r, tq);
         if (k >= ninner[L.line]) continue;
-        const LineBC bc = line_bc(g, T, dir, p, r, tq, Gc);
+        const LineBC bc = line_bc_lds(g, T, dir, L.line);
         const bool has = (L.ix > 0) || (bc.lt == VERB_BC_PERIODIC);
         if (!has) continue;
         const double cr = FC[t];
@@ -316,7 +333,7 @@ __device__ void quickest_step(const verb_grid& g, const Tile& T, int dir, int k,
         LineGeom L = line_of(T, dir, p, r, tq);
         if (run && k >= ninner[L.line]) run = false;
         if (!run) { Y[t] = X[t]; continue; }
-        const LineBC bc = line_bc(g, T, dir, p, r, tq, Gc);
+        const LineBC bc = line_bc_lds(g, T, dir, L.line);
         const int n = L.n;
         int i = L.ix;
         if (i == n - 1) {
@@ -348,6 +365,8 @@ __global__ __launch_bounds__(kThreads) void convection_kernel(const verb_grid g,
     extern __shared__ double lds[];
     Tile T;
     T.nP = g.nP; T.nR = g.nR; T.TQ = TQ; T.npt = g.nP * g.nR * TQ;
+    T.tq_shift = __builtin_ctz((unsigned)TQ);
+    T.mR = g.nR > 1 ? 0xffffffffu / (unsigned)g.nR + 1u : 0u;
     T.NQ = (long)g.nV * g.nK;
     T.q0 = (long)tile * TQ;
     const int npt = T.npt;
@@ -362,6 +381,9 @@ __global__ __launch_bounds__(kThreads) void convection_kernel(const verb_grid g,
     int* ninR = ninP + g.nR * TQ;           // [nP*TQ]
     int* nsteps = ninR + g.nP * TQ;         // [TQ]
     int* active = nsteps + TQ;              // [TQ]
+    const int nint = (g.nR + g.nP) * TQ + 2 * TQ;  // ints above
+    double* bcl = (double*)(ninP + nint + (nint & 1));  // [(nR + nP) * TQ][4], 8-byte aligned
+    T.bc = bcl;
 
     const int tid = threadIdx.x;
     const long NQ = T.NQ;
@@ -448,6 +470,15 @@ __global__ __launch_bounds__(kThreads) void convection_kernel(const verb_grid g,
             }
         }
         (isP ? ninP : ninR)[ll] = nin;
+        // the line's boundary data, read by every sub-step
+        if (q < NQ) {
+            const LineBC bc = isP ? line_bc(g, T, 0, 0, a, tq, g.G_conv) : line_bc(g, T, 1, a, 0, tq, g.G_conv);
+            double* e = bcl + 4 * l;
+            e[0] = bc.lbc;
+            e[1] = bc.ubc;
+            e[2] = bc.sl;
+            e[3] = bc.su;
+        }
     }
     __syncthreads();
 
@@ -554,7 +585,9 @@ __global__ __launch_bounds__(kThreads) void convection_kernel(const verb_grid g,
 size_t convection_lds_bytes(int nP, int nR, int TQ) {
     const size_t npt = (size_t)nP * nR * TQ;
     size_t b = (5 * npt + TQ + 2 * kThreads) * sizeof(double);
-    b += (size_t)(nR * TQ + nP * TQ + 2 * TQ) * sizeof(int);
+    const size_t nint = (size_t)(nR + nP) * TQ + 2 * TQ;
+    b += (nint + (nint & 1)) * sizeof(int);
+    b += (size_t)(nR + nP) * TQ * 4 * sizeof(double);
     return b;
 }
 
