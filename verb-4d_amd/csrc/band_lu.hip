@@ -414,7 +414,6 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
                     constexpr int C = Cc;
                     if (!(fast && C == c && C < nc)) return;
                     const int j = j0 + C;
-                    const int km = min(kl, n - 1 - j);
                     T xc[NQ];
                     bool cd[NQ];
                     unsigned kk[NQ];
@@ -422,8 +421,12 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
                     static_for<0, NQ>([&](auto qc) {
                         constexpr int q = qc;
                         xc[q] = x[q][C];
-                        // candidates: logical rows C..C+km
-                        cd[q] = (pos[q] >= C) & (pos[q] <= C + km);
+                        // candidates: logical rows C..C+km. The rows past
+                        // C+km have not moved and hold exact zeros in column
+                        // C (below the band, or past the last row), so they
+                        // neither reach a nonzero maximum key nor change a
+                        // value when they take a zero multiplier.
+                        cd[q] = pos[q] >= C;
                         kk[q] = cd[q] ? P::key(xc[q]) : 0u;
                         mloc = max(mloc, kk[q]);
                     });
