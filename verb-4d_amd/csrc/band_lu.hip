@@ -460,8 +460,9 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
                     const int jpos = __builtin_amdgcn_readlane(posp, pl);
                     const int jp = jpos - C;
                     ju = max(ju, min(j + ku + jp, n - 1));
-                    myjp = (l == C) ? jp : myjp;
-                    myju = (l == C) ? ju : myju;
+                    // (jp and ju are uniform: written into lane C)
+                    asm("v_writelane_b32 %0, %1, %2" : "+v"(myjp) : "s"(jp), "n"(C));
+                    asm("v_writelane_b32 %0, %1, %2" : "+v"(myju) : "s"(ju), "n"(C));
                     // pivot row right of the step (the branches end
                     // differently so that the compiler keeps them apart
                     // instead of merging them through a copy of a row set)
