@@ -868,16 +868,33 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
                 la[kk] = S.linv[jj * NB + 4 * kk + g];
                 dsl[kk] = (4 * kk + g < nc) ? S.dslot[4 * kk + g] : -1;
             }
-            for (int cb = lb - 1; cb < CB - 1; cb += RB - 1) {
-                A4 d = {T(0), T(0), T(0), T(0)};
+            // this wavefront's column blocks lb-1, lb-1+(RB-1), ...: their
+            // MFMA chains are interleaved (a chain is four dependent MFMAs)
+            constexpr int NBW = (CB - 1 + RB - 2) / (RB - 1);
+            A4 d[NBW];
+            T bv[NBW][4];
+            static_for<0, NBW>([&](auto tc) {
+                constexpr int t = tc;
+                const int cb = lb - 1 + t * (RB - 1);
+                const int cbr = (cb < CB - 1) ? cb : lb - 1;
 #pragma unroll
-                for (int kk = 0; kk < 4; kk++) {
-                    const T bv = (dsl[kk] >= 0) ? S.sb[dsl[kk] * W + NB * cb + jj] : T(0);
-                    d = P::mfma(la[kk], bv, d);
-                }
+                for (int kk = 0; kk < 4; kk++)
+                    bv[t][kk] = (dsl[kk] >= 0) ? S.sb[dsl[kk] * W + NB * cbr + jj] : T(0);
+                d[t] = A4{T(0), T(0), T(0), T(0)};
+            });
 #pragma unroll
-                for (int r = 0; r < 4; r++) S.u12[P::crow(g, r) * W + NB * cb + jj] = d[r];
-            }
+            for (int kk = 0; kk < 4; kk++)
+                static_for<0, NBW>([&](auto tc) {
+                    constexpr int t = tc;
+                    d[t] = P::mfma(la[kk], bv[t][kk], d[t]);
+                });
+            static_for<0, NBW>([&](auto tc) {
+                constexpr int t = tc;
+                const int cb = lb - 1 + t * (RB - 1);
+                if (cb < CB - 1)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) S.u12[P::crow(g, r) * W + NB * cb + jj] = d[t][r];
+            });
         } else if (lb != 0) {
             const int l = lane_fresh();
             for (int t = NB * 4 * (lb - 1) + l; t < W; t += 64 * (RB - 1)) {
