@@ -393,7 +393,7 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
                     chk[k & 1] = fma(x[q][k], T(0), chk[k & 1]);
                 });
                 pos[q] = h ? row : kNone;
-                const T r = S.rw[(j0 + rc) % PR];
+                const T r = S.rw[(unsigned)(j0 + rc) % PR];
                 rh[q] = h ? r : T(0);
                 chk[0] = fma(rh[q], T(0), chk[0]);
             });
@@ -632,12 +632,12 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
             static_for<0, NQ>([&](auto qc) {
                 constexpr int q = qc;
                 const int row = l + 64 * q;
-                if (row < PR) {
+                if (64 * (q + 1) <= PR || row < PR) {
                     static_for<0, NB>([&](auto kc) {
                         constexpr int k = kc;
                         pn[k * PR + pos[q]] = x[q][k];
                     });
-                    S.rw[(j0 + pos[q]) % PR] = rh[q];
+                    S.rw[(unsigned)(j0 + pos[q]) % PR] = rh[q];
                     const int sl = (row < NB) ? row : ((pos[q] != row) ? NB + pos[q] : -1);
                     S.slot[row] = sl;
                     S.dslot[pos[q]] = (pos[q] < NB || pos[q] != row) ? sl : -1;
