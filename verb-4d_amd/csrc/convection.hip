@@ -75,6 +75,40 @@ __device__ inline double unl_at(const double* X, const double* Y, int base, int 
     return v;
 }
 
+// Points j0 .. j0+N-1 of a line with the ghost rules of orig_at (Y == X)
+// or unl_at (lower ghosts from X, the line and upper ghosts from Y). The
+// two boundary kinds of the common grids (periodic at both ends, constant
+// value at both ends) gather without per-point branches; any other pair
+// goes through the point rules above. (n >= 6, so a periodic ghost within
+// five points of an end is a point of the line.)
+template <int N>
+__device__ inline void gather(const double* X, const double* Y, int base, int stride, int n, int j0,
+                              const LineBC& bc, double (&out)[N]) {
+    if (bc.lt == VERB_BC_PERIODIC && bc.ut == VERB_BC_PERIODIC) {
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            const int j = j0 + i;
+            const int jj = (j < 0) ? j + n - 1 : ((j >= n) ? j - (n - 1) : j);
+            const double* src = (j < 0) ? X : Y;
+            out[i] = src[base + jj * stride];
+        }
+    } else if (bc.lt == VERB_BC_CONSTANT_VALUE && bc.ut == VERB_BC_CONSTANT_VALUE) {
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            const int j = j0 + i;
+            const int jc = min(max(j, 0), n - 1);
+            const double v = Y[base + jc * stride];
+            out[i] = (j < 0) ? bc.lbc : ((j >= n) ? bc.ubc : v);
+        }
+    } else if (X == Y) {
+#pragma unroll
+        for (int i = 0; i < N; i++) out[i] = orig_at(X, base, stride, n, j0 + i, bc);
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; i++) out[i] = unl_at(X, Y, base, stride, n, j0 + i, bc);
+    }
+}
+
 // Order-9 unlimited face value (QUICKEST6.cpp:192-239).
 __device__ inline double unlimited_face(const double W[10], double courant) {
     const double cs = courant * courant;
@@ -264,8 +298,7 @@ __device__ void quickest_step(const verb_grid& g, const Tile& T, int dir, int k,
         const bool has = (L.ix > 0) || (bc.lt == VERB_BC_PERIODIC);
         if (!has) continue;
         double W[10];
-#pragma unroll
-        for (int i = 0; i < 10; i++) W[i] = orig_at(X, L.base, L.stride, L.n, L.ix - 5 + i, bc);
+        gather<10>(X, X, L.base, L.stride, L.n, L.ix - 5, bc, W);
         F[t] = unlimited_face(W, FC[t]);
     }
     __syncthreads();
@@ -310,18 +343,14 @@ __device__ void quickest_step(const verb_grid& g, const Tile& T, int dir, int k,
         if (!has) continue;
         const double cr = FC[t];
         double UW[10];
-#pragma unroll
-        for (int i = 0; i < 10; i++) UW[i] = unl_at(X, Y, L.base, L.stride, L.n, L.ix - 5 + i, bc);
-        double ou, oc, od;
-        if (cr >= 0) {
-            ou = orig_at(X, L.base, L.stride, L.n, L.ix - 2, bc);
-            oc = orig_at(X, L.base, L.stride, L.n, L.ix - 1, bc);
-            od = orig_at(X, L.base, L.stride, L.n, L.ix, bc);
-        } else {
-            od = orig_at(X, L.base, L.stride, L.n, L.ix - 1, bc);
-            oc = orig_at(X, L.base, L.stride, L.n, L.ix, bc);
-            ou = orig_at(X, L.base, L.stride, L.n, L.ix + 1, bc);
-        }
+        gather<10>(X, Y, L.base, L.stride, L.n, L.ix - 5, bc, UW);
+        // original points ix-2 .. ix+1: (ou, oc, od) is (ix-2, ix-1, ix)
+        // for a non-negative Courant number, else (ix+1, ix, ix-1)
+        double o4[4];
+        gather<4>(X, X, L.base, L.stride, L.n, L.ix - 2, bc, o4);
+        const double ou = (cr >= 0) ? o4[0] : o4[3];
+        const double oc = (cr >= 0) ? o4[1] : o4[2];
+        const double od = (cr >= 0) ? o4[2] : o4[1];
         F[t] = limited_face(UW, ou, oc, od, F[t], cr);
     }
     __syncthreads();
