@@ -6,9 +6,9 @@
 // j..j+kl of column j; ipiv 1-based; U stored in rows 0..kl+ku of the LAPACK
 // band column, the multipliers in rows kl+ku+1.. with the dgbtf2 convention
 // (each column's multipliers in the row order of its own step). The
-// right-hand side B0*psd + C0 is forward-eliminated on the fly, in
-// dgbtrs order (swap, then subtract the column's multipliers), so the plain
-// path never stores L.
+// right-hand side B0*psd + C0 is forward-eliminated panel by panel, in
+// dgbtrs order (swap, then subtract the column's multipliers unless b(j) is
+// zero), so the plain path never stores L.
 //
 // Blocking. The matrix is energy-fast (kl = ku = nV+1). Panels are NB = 16
 // columns wide. At panel p the live window is RB row blocks x CB column
@@ -21,8 +21,8 @@
 //
 // Look-ahead schedule, four barriers per panel:
 //   stage p: one wavefront (the one that held the panel rows of p-1) factors
-//      panel p in registers with full-row interchanges and eliminates the
-//      right-hand-side window, while the others finish the trailing update
+//      panel p in registers with full-row interchanges, while the others
+//      finish the trailing update
 //      of panel p-1 on MFMA, stream its U rows to HBM and fetch the factoring
 //      wavefront's new row block into an LDS stash;
 //   3a: the rows that the panel's interchanges move are staged to LDS, and
@@ -30,7 +30,8 @@
 //   3b/3c: U12 = L11^-1 P A12 on MFMA; moved rows below the panel take their
 //      new contents;
 //   4a: column block p+1 is updated first and written to the other panel
-//      buffer: it is panel p+1.
+//      buffer: it is panel p+1; the wavefront that held the panel rows
+//      eliminates the right-hand-side window with panel p's multipliers.
 // U goes out in the LAPACK band layout (and, for the cached path, with the
 // multipliers re-ordered to the dgbtf2 convention) or, when the factors are
 // not kept, in contiguous 16-row blocks that the back-substitution streams.
@@ -258,6 +259,7 @@ struct Lds {
     T linv[NB * NB];                         // L11^-1 of the current panel, row-major
     int slot[PR];                            // sb row holding window row i's old contents, -1: none
     int dslot[PR];                           // sb row whose contents window row i takes, -1: unchanged
+    int perm[2][PR];                         // per panel parity: window row that the interchanges put at each position
     int jp[NB];
     int ju, info;
     int lfin;                                // the panel's multipliers are all finite
@@ -376,7 +378,6 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
             // they live in registers)
             T x[NQ][NB];
             int pos[NQ];
-            T rh[NQ];
             // (loads from a clamped row, so that they need no branch; a
             // non-finite entry turns the sum of the entries times zero into
             // NaN)
@@ -393,9 +394,6 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
                     chk[k & 1] = fma(x[q][k], T(0), chk[k & 1]);
                 });
                 pos[q] = h ? row : kNone;
-                const T r = S.rw[(unsigned)(j0 + rc) % PR];
-                rh[q] = h ? r : T(0);
-                chk[0] = fma(rh[q], T(0), chk[0]);
             });
             const bool bad = !isfinite(chk[0] + chk[1]);
             // Columns are taken by an unrolled sequence with the column index
@@ -467,7 +465,6 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
                     // differently so that the compiler keeps them apart
                     // instead of merging them through a copy of a row set)
                     T pr[NB];
-                    T prh = T(0);
                     static_for<0, NQ>([&](auto qc) {
                         constexpr int q = qc;
                         if (qp == q) {
@@ -475,7 +472,6 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
                                 constexpr int k = kc;
                                 pr[k] = P::readlane(x[q][k], pl);
                             });
-                            prh = P::readlane(rh[q], pl);
                             asm volatile("; pivot row in set %0" ::"n"(q));
                         }
                     });
@@ -490,7 +486,6 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
                         x[q][C] = e ? Lq : xc[q];
                         // finite values: fma(0, t, v) == v for the rows the
                         // step does not touch
-                        rh[q] = fma(Lq, -prh, rh[q]);
                         static_for<C + 1, NB>([&](auto kc) {
                             constexpr int k = kc;
                             x[q][k] = fma(Lq, -pr[k], x[q][k]);
@@ -562,19 +557,15 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
                 });
                 const int jp = jpos - c;
                 myjp = (l == c) ? jp : myjp;
-                T piv = T(0), prh = T(0);
+                T piv = T(0);
                 static_for<0, NQ>([&](auto qc) {
                     constexpr int q = qc;
-                    if (q == qp) {
-                        piv = P::readlane(xc[q], pl);
-                        prh = P::readlane(rh[q], pl);
-                    }
+                    if (q == qp) piv = P::readlane(xc[q], pl);
                 });
                 if (piv != T(0)) {
                     ju = max(ju, min(j + ku + jp, n - 1));
                     const int prow = pl + 64 * qp;
                     const T rinv = T(1) / piv;
-                    const T tb2 = -prh;
                     const bool finite = fabs(rinv) <= P::maxfinite;
                     nonfin |= !finite;
                     T Lv[NQ];
@@ -588,7 +579,6 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
                         const T Lq = e ? xc[q] * rinv : T(0);
                         const int row = l + 64 * q;
                         if (row < PR) pn[c * PR + row] = e ? Lq : xc[q];
-                        if (tb2 != T(0)) rh[q] = e ? fma(Lq, tb2, rh[q]) : rh[q];
                         Lv[q] = Lq;
                         ev[q] = e;
                     });
@@ -637,7 +627,7 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
                         constexpr int k = kc;
                         pn[k * PR + pos[q]] = x[q][k];
                     });
-                    S.rw[(unsigned)(j0 + pos[q]) % PR] = rh[q];
+                    S.perm[p & 1][pos[q]] = row;
                     const int sl = (row < NB) ? row : ((pos[q] != row) ? NB + pos[q] : -1);
                     S.slot[row] = sl;
                     S.dslot[pos[q]] = (pos[q] < NB || pos[q] != row) ? sl : -1;
@@ -708,6 +698,59 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
 #pragma unroll
                     for (int r = 0; r < 4; r++) S.stash[(sft * 4 + r) * 64 + l] = ft[t][r];
             });
+            // Right-hand-side window of panel q, on the wavefront that holds
+            // the next panel rows (it has the least trailing-update work):
+            // the panel's interchanges, then per column c the multipliers
+            // times b(c), skipped when b(c) is zero (dgbtrs). Window
+            // position i holds the row that the interchanges put there; its
+            // multipliers in column c sit at position i of the panel buffer
+            // (rows past the band hold zeros). The panel rows' slots then
+            // take the rows that enter the window of panel q+1.
+            if (lb == 1 && rhs) {
+                constexpr int NQ = (PR + 63) / 64;
+                const int* const perm = S.perm[q & 1];
+                T rq[NQ];
+                static_for<0, NQ>([&](auto qc) {
+                    constexpr int q = qc;
+                    const int i = l + 64 * q;
+                    const bool h = 64 * (q + 1) <= PR || i < PR;
+                    const int src = perm[h ? i : 0];
+                    const T v = S.rw[(unsigned)(j0 + src) % PR];
+                    rq[q] = h ? v : T(0);
+                });
+                // (the multipliers of all columns are requested first)
+                T lm[NB][NQ];
+                static_for<0, NB>([&](auto cc) {
+                    constexpr int c = cc;
+                    static_for<0, NQ>([&](auto qc) {
+                        constexpr int q = qc;
+                        const int i = l + 64 * q;
+                        const bool h = 64 * (q + 1) <= PR || i < PR;
+                        lm[c][q] = pn[c * PR + (h ? i : 0)];
+                    });
+                });
+                static_for<0, NB>([&](auto cc) {
+                    constexpr int c = cc;
+                    const T bc = P::readlane(rq[0], c);  // position c < 16: lane c
+                    if (c < nc && bc != T(0)) {
+                        static_for<0, NQ>([&](auto qc) {
+                            constexpr int q = qc;
+                            const int i = l + 64 * q;
+                            const bool h = 64 * (q + 1) <= PR || i < PR;
+                            rq[q] = (h && i > c) ? fma(lm[c][q], -bc, rq[q]) : rq[q];
+                        });
+                    }
+                });
+                static_for<0, NQ>([&](auto qc) {
+                    constexpr int q = qc;
+                    const int i = l + 64 * q;
+                    if (64 * (q + 1) <= PR || i < PR) S.rw[(unsigned)(j0 + i) % PR] = rq[q];
+                });
+                // the panel rows' entries are final
+                if (yv && l < nc) yv[j0 + l] = rq[0];
+                const int ie = j0 + PR + l;
+                if (l < NB) S.rw[(unsigned)ie % PR] = (ie < n) ? rhs[ie] : T(0);
+            }
             // U rows j0..j0+nc-1: entry e is row e%16 at column j0 + e/16, up
             // to kl+ku past the diagonal (the entries past the row's fill
             // extent are exact zeros in the window). 16 consecutive lanes
@@ -830,26 +873,14 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
                 }
             }
         }
-        if (lb == 0) {
-            const int l = lane_fresh();
-            // the forward-eliminated right-hand side of the panel rows is final
-            if (yv)
-                for (int c = l; c < nc; c += 64) yv[j0 + c] = S.rw[(j0 + c) % PR];
-        }
         __syncthreads();
-        // right-hand-side entry of the row block that enters the window
-        T rv = T(0);
         if (lb == 0) {
             // The panel rows' registers are free: the row block that enters
             // the window at the bottom for panel p+1 belongs to this
-            // wavefront. Its tile in column block p+1 and its right-hand-side
-            // entries are requested while the other wavefronts form U12
-            // (nothing waits for them before step 4a, where the entries take
-            // the panel rows' rw slots).
+            // wavefront. Its tile in column block p+1 is requested while the
+            // other wavefronts form U12 (nothing waits for it before 4a).
             const int l = lane_fresh(), g = l >> 4, jj = l & 15;
             const int rb = p + RB;
-            const int i = NB * rb + l;
-            rv = (l < NB && rhs && i < n) ? rhs[i] : T(0);
             // slot 0 of the layout of panel p+1 (column block p+1); the
             // other slots come through the stash while panel p+1 is factored
             acc[0] = load_tile<T>(rs, n, nx, rb, p + 1, jj, g);
@@ -985,7 +1016,6 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
                         ab[(size_t)(j0 + c) * ldab + kv + 1 + i] = (i < km) ? cp[c * PR + c + 1 + i] : T(0);
                 }
             }
-            if (l < NB) S.rw[(NB * (p + RB) + l) % PR] = rv;
             if (p + 1 < npan) {
 #pragma unroll
                 for (int r = 0; r < 4; r++) pnx[jj * PR + NB * (RB - 1) + P::crow(g, r)] = acc[0][r];
