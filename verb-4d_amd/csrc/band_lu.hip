@@ -262,6 +262,7 @@ struct Lds {
     int perm[2][PR];                         // per panel parity: window row that the interchanges put at each position
     int jp[NB];
     int ju, info;
+    unsigned jz;                             // columns of the current panel with a zero pivot (bit c)
     int lfin;                                // the panel's multipliers are all finite
     int qnode;                               // persistent mode: current dispatch slot
 };
@@ -371,8 +372,8 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
             // others come from the stash after the stage barrier
 #pragma unroll
             for (int s = 0; s < CB; s++) acc[s] = __builtin_nondeterministic_value(acc[s]);
-            int ju = S.ju;
             int info = S.info;
+            unsigned zmask = 0;  // (uniform) columns with a zero pivot
             constexpr int kNone = 1 << 20;
             // (all indices into these arrays are compile-time constants, so
             // they live in registers)
@@ -405,7 +406,7 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
             // records the interchange and the U fill extent after the step.
             const bool fast = __ballot(bad) == 0;
             bool nonfin = !fast;  // (multipliers may be non-finite)
-            int myjp = 0, myju = 0;
+            int myjp = 0;
             int c = 0;
             while (c < nc) {
                 static_for<0, NB>([&](auto Cc) {
@@ -457,10 +458,8 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
                     if (!(piv != T(0) && fabs(rinv) <= P::maxfinite)) return;
                     const int jpos = __builtin_amdgcn_readlane(posp, pl);
                     const int jp = jpos - C;
-                    ju = max(ju, min(j + ku + jp, n - 1));
-                    // (jp and ju are uniform: written into lane C)
+                    // (jp is uniform: written into lane C)
                     asm("v_writelane_b32 %0, %1, %2" : "+v"(myjp) : "s"(jp), "n"(C));
-                    asm("v_writelane_b32 %0, %1, %2" : "+v"(myju) : "s"(ju), "n"(C));
                     // pivot row right of the step (the branches end
                     // differently so that the compiler keeps them apart
                     // instead of merging them through a copy of a row set)
@@ -563,7 +562,6 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
                     if (q == qp) piv = P::readlane(xc[q], pl);
                 });
                 if (piv != T(0)) {
-                    ju = max(ju, min(j + ku + jp, n - 1));
                     const int prow = pl + 64 * qp;
                     const T rinv = T(1) / piv;
                     const bool finite = fabs(rinv) <= P::maxfinite;
@@ -600,8 +598,8 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
                     }
                 } else {
                     if (info == 0) info = j + 1;
+                    zmask |= 1u << c;
                 }
-                myju = (l == c) ? ju : myju;
                 wave_sync();
                 static_for<0, NQ>([&](auto qc) {
                     constexpr int q = qc;
@@ -633,15 +631,10 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
                     S.dslot[pos[q]] = (pos[q] < NB || pos[q] != row) ? sl : -1;
                 }
             });
-            if (l < NB) {
-                S.jp[l] = (l < nc) ? myjp : 0;
-                if (l < nc) {
-                    ipiv[j0 + l] = (int64_t)(j0 + l + myjp + 1);
-                    if (uext) uext[j0 + l] = myju;
-                }
-            }
+            // (ipiv and the fill extents are written from these in step 3a)
+            if (l < NB) S.jp[l] = (l < nc) ? myjp : 0;
             if (l == 0) {
-                S.ju = ju;
+                S.jz = zmask;
                 S.info = info;
                 S.lfin = !nonfin;
             }
@@ -790,7 +783,6 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
         __syncthreads();
         if (p == npan) break;
         const int lb = (w - p % RB + RB) % RB;  // local row block of this wavefront
-        const int ju = S.ju;
         const bool lfin = S.lfin != 0;
         if (fac) {
             const int l = lane_fresh();
@@ -858,6 +850,29 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
                 for (int r = 0; r < 4; r++) acc[s][r] = S.stash[(s * 4 + r) * 64 + l];
         }
 
+        // the panel's ipiv and U fill extents (dgbtf2: ju = max(ju, min(j +
+        // ku + jp, n-1)) after each column with a nonzero pivot), on the
+        // wavefront that holds the panel rows: lane c takes column c, the
+        // running maximum is a prefix maximum over the 16 lanes
+        if (lb == 0) {
+            const int l = lane_fresh();
+            const int c = l & (NB - 1);
+            const int jpc = S.jp[c];
+            const unsigned zm = S.jz;
+            const int jprev = S.ju;
+            int v = (c < nc && !((zm >> c) & 1u)) ? min(j0 + c + ku + jpc, n - 1) : 0;
+            v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false));  // row_shr:1
+            v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false));  // row_shr:2
+            v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false));  // row_shr:4
+            v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false));  // row_shr:8
+            const int uj = max(jprev, v);
+            if (l < nc) {
+                ipiv[j0 + l] = (int64_t)(j0 + l + jpc + 1);
+                if (uext) uext[j0 + l] = uj;
+            }
+            const int jnew = __builtin_amdgcn_readlane(uj, nc - 1);
+            if (l == 0) S.ju = jnew;
+        }
         // 3a. pre-interchange contents of the rows that move -> sb (columns
         //     right of the panel)
         {
@@ -973,6 +988,7 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
         //     window of p+1 (the wavefront that held the panel rows brings the
         //     row block that entered the window, in slot 0 of its new layout)
         T* const pnx = S.pnl[(p + 1) & 1];
+        const int ju = S.ju;  // (formed in step 3a)
         if (lb != 0) {
             if (p + 1 < npan) {
                 const int l = lane_fresh(), g = l >> 4, jj = l & 15;
