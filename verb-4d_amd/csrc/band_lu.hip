@@ -271,8 +271,9 @@ struct Lds {
 
 __global__ void zero_counter(int* c) { *c = 0; }
 
+// (FP32 windows up to 8x14 fit 128 registers: two workgroups per CU)
 template <typename T, int RB, int CB>
-__global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
+__global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof(T) == 4 && RB <= 8 && CB <= 14) ? 4 : 1))) void band_lu_window(LuArgsT<T> a) {
     using L = Lds<T, RB, CB>;
     using P = Prec<T>;
     using A4 = typename P::acc4;
@@ -803,18 +804,27 @@ __global__ __launch_bounds__(RB * 64, 1) void band_lu_window(LuArgsT<T> a) {
                 // column i starts on a 16-byte boundary)
                 constexpr int E = 16 / sizeof(T);
                 using V16 = T __attribute__((ext_vector_type(E)));
+                // (FP32 keeps element reads: its register budget is that of
+                // two workgroups per CU)
                 auto fetch = [&](auto i0c, auto i1c) {
                     static_for<decltype(i0c)::value, decltype(i1c)::value>([&](auto ic) {
                         constexpr int i = ic;
-                        static_for<(i + 1) / E, NB / E>([&](auto vc) {
-                            constexpr int cv = E * vc;
-                            const V16 t = *reinterpret_cast<const V16*>(pn + i * PR + cv);
-                            static_for<0, E>([&](auto ec) {
-                                constexpr int e = ec;
-                                constexpr int c = cv + e;
-                                if constexpr (c > i) lc[i * NB + c] = t[e];
+                        if constexpr (sizeof(T) == 8) {
+                            static_for<(i + 1) / E, NB / E>([&](auto vc) {
+                                constexpr int cv = E * vc;
+                                const V16 t = *reinterpret_cast<const V16*>(pn + i * PR + cv);
+                                static_for<0, E>([&](auto ec) {
+                                    constexpr int e = ec;
+                                    constexpr int c = cv + e;
+                                    if constexpr (c > i) lc[i * NB + c] = t[e];
+                                });
                             });
-                        });
+                        } else {
+                            static_for<i + 1, NB>([&](auto cc) {
+                                constexpr int c = cc;
+                                lc[i * NB + c] = pn[i * PR + c];
+                            });
+                        }
                     });
                 };
                 auto eliminate = [&](auto i0c, auto i1c) {
