@@ -22,16 +22,17 @@
 // Look-ahead schedule, four barriers per panel:
 //   stage p: one wavefront (the one that held the panel rows of p-1) factors
 //      panel p in registers with full-row interchanges, while the others
-//      finish the trailing update
-//      of panel p-1 on MFMA, stream its U rows to HBM and fetch the factoring
-//      wavefront's new row block into an LDS stash;
-//   3a: the rows that the panel's interchanges move are staged to LDS, and
-//      the factoring wavefront forms L11^-1;
+//      finish the trailing update of panel p-1 on MFMA, stream its U rows
+//      to HBM and fetch the factoring wavefront's new row block into an LDS
+//      stash; the one that holds the next panel rows also forward-eliminates
+//      panel p-1's right-hand-side window;
+//   3a: the rows that the panel's interchanges move are staged to LDS, the
+//      factoring wavefront forms L11^-1, and the panel-row wavefront writes
+//      ipiv and the U fill extents;
 //   3b/3c: U12 = L11^-1 P A12 on MFMA; moved rows below the panel take their
 //      new contents;
 //   4a: column block p+1 is updated first and written to the other panel
-//      buffer: it is panel p+1; the wavefront that held the panel rows
-//      eliminates the right-hand-side window with panel p's multipliers.
+//      buffer: it is panel p+1.
 // U goes out in the LAPACK band layout (and, for the cached path, with the
 // multipliers re-ordered to the dgbtf2 convention) or, when the factors are
 // not kept, in contiguous 16-row blocks that the back-substitution streams.
@@ -404,7 +405,7 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
             // candidate is unique in its key and the pivot is nonzero with a
             // finite reciprocal. Any other column takes the general step
             // below, and the sequence resumes after it. Per column c, lane c
-            // records the interchange and the U fill extent after the step.
+            // records the interchange; columns with a zero pivot are marked.
             const bool fast = __ballot(bad) == 0;
             bool nonfin = !fast;  // (multipliers may be non-finite)
             int myjp = 0;
