@@ -348,6 +348,9 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
     }
     __syncthreads();
     int ju_prev = 0;
+    // sb rows whose contents this wavefront's rows r take in slots 2..CB-1,
+    // applied in the next stage's trailing update (-1: none)
+    int dfx[4] = {-1, -1, -1, -1};
     for (int p = 0;; p++) {
         const int j0 = NB * p;
         const int nc = min(NB, n - j0);
@@ -667,6 +670,15 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
             T an[4];
 #pragma unroll
             for (int kk = 0; kk < 4; kk++) an[kk] = -pn[(4 * kk + g) * PR + rowbase + jj];
+            // the contents that rows moved by panel q's interchanges take in
+            // slots 2..CB-1 (slot 1 took them in step 3c)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                if (dfx[r] >= 0) {
+#pragma unroll
+                    for (int s = 2; s < CB; s++) acc[s][r] = S.sb[dfx[r] * W + NB * (s - 1) + jj];
+                }
+            }
             // slot 1 was updated in step 4a
 #pragma unroll
             for (int s = 2; s < CB; s++) {
@@ -978,19 +990,31 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
                 });
             }
         }
-        // 3c. rows below the panel whose contents changed take them from sb
+        // 3c. rows below the panel whose contents changed take them from sb:
+        //     slot 1 now (step 4a updates it); slots 2..CB-1 at the start of
+        //     the next stage's trailing update, which is the next to use them
+        //     (sb is rewritten in the next step 3a, or, when L is kept, as
+        //     scratch in step 4a: then all slots now)
         if (lb != 0) {
             const int l = lane_fresh(), g = l >> 4, jj = l & 15;
-            int ds[4];
 #pragma unroll
-            for (int r = 0; r < 4; r++) ds[r] = S.dslot[NB * lb + P::crow(g, r)];
+            for (int r = 0; r < 4; r++) dfx[r] = S.dslot[NB * lb + P::crow(g, r)];
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                if (ds[r] >= 0) {
+            for (int r = 0; r < 4; r++)
+                if (dfx[r] >= 0) acc[1][r] = S.sb[dfx[r] * W + jj];
+            if (a.write_L) {
 #pragma unroll
-                    for (int s = 1; s < CB; s++) acc[s][r] = S.sb[ds[r] * W + NB * (s - 1) + jj];
+                for (int r = 0; r < 4; r++) {
+                    if (dfx[r] >= 0) {
+#pragma unroll
+                        for (int s = 2; s < CB; s++) acc[s][r] = S.sb[dfx[r] * W + NB * (s - 1) + jj];
+                    }
+                    dfx[r] = -1;
                 }
             }
+        } else {
+#pragma unroll
+            for (int r = 0; r < 4; r++) dfx[r] = -1;
         }
         __syncthreads();
 
