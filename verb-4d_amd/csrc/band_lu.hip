@@ -252,7 +252,11 @@ template <typename T, int RB, int CB>
 struct Lds {
     static constexpr int W = NB * (CB - 1);  // window columns right of the panel
     static constexpr int PR = NB * RB;       // window rows
-    T pnl[2][NB * PR];                       // panels p and p+1, column-major (full-row interchanges)
+    // column stride of the panel buffers: 4 doubles of padding put the 16
+    // columns that one 16x16 tile writes on different LDS banks (and keep
+    // every column start 16-byte aligned)
+    static constexpr int PS = PR + 4;
+    T pnl[2][NB * PS];                       // panels p and p+1, column-major (full-row interchanges)
     T u12[NB * W];
     T sb[2 * NB * W];                        // pre-interchange contents of the rows that move
     T stash[CB * 4 * 64];                    // wavefront 0's accumulators while it factors
@@ -280,6 +284,7 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
     using A4 = typename P::acc4;
     constexpr int W = L::W;
     constexpr int PR = L::PR;
+    constexpr int PS = L::PS;
     static_assert(RB >= 3, "the trailing update's U rows are written by row blocks 2..RB-1");
     extern __shared__ double lds_raw[];  // (8-byte aligned)
     L& S = *reinterpret_cast<L*>(lds_raw);
@@ -339,7 +344,7 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
         //    panel 0 keeps its other tiles in the stash meanwhile
         const int l = lane_fresh(), g = l >> 4, jj = l & 15;
 #pragma unroll
-        for (int r = 0; r < 4; r++) S.pnl[0][jj * PR + NB * w + P::crow(g, r)] = acc[0][r];
+        for (int r = 0; r < 4; r++) S.pnl[0][jj * PS + NB * w + P::crow(g, r)] = acc[0][r];
         if (w == RB - 1)
 #pragma unroll
             for (int s = 1; s < CB; s++)
@@ -395,7 +400,7 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
                 const int rc = h ? row : PR - 1;
                 static_for<0, NB>([&](auto kc) {
                     constexpr int k = kc;
-                    const T v = pn[k * PR + rc];
+                    const T v = pn[k * PS + rc];
                     x[q][k] = h ? v : T(0);
                     chk[k & 1] = fma(x[q][k], T(0), chk[k & 1]);
                 });
@@ -507,7 +512,7 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
                     if (row < PR)
                         static_for<0, NB>([&](auto kc) {
                             constexpr int k = kc;
-                            pn[k * PR + row] = x[q][k];
+                            pn[k * PS + row] = x[q][k];
                         });
                 });
                 wave_sync();
@@ -519,7 +524,7 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
                 static_for<0, NQ>([&](auto qc) {
                     constexpr int q = qc;
                     const int row = l + 64 * q;
-                    xc[q] = (row < PR) ? pn[c * PR + row] : T(0);
+                    xc[q] = (row < PR) ? pn[c * PS + row] : T(0);
                     // candidates: logical rows c..c+km
                     cand[q] = (pos[q] >= c) & (pos[q] <= c + km);
                     v[q] = cand[q] ? fabs(xc[q]) : T(-1);
@@ -581,23 +586,23 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
                         // multipliers; 0 for rows the step does not touch
                         const T Lq = e ? xc[q] * rinv : T(0);
                         const int row = l + 64 * q;
-                        if (row < PR) pn[c * PR + row] = e ? Lq : xc[q];
+                        if (row < PR) pn[c * PS + row] = e ? Lq : xc[q];
                         Lv[q] = Lq;
                         ev[q] = e;
                     });
                     // rank-1 update right of the step; the pivot row's own
                     // entries stay as they are (its multiplier is 0)
                     for (int k = c + 1; k < NB; k++) {
-                        const T t = -pn[k * PR + prow];
+                        const T t = -pn[k * PS + prow];
                         static_for<0, NQ>([&](auto qc) {
                             constexpr int q = qc;
                             const int row = l + 64 * q;
                             if (row < PR) {
-                                const T xv = pn[k * PR + row];
+                                const T xv = pn[k * PS + row];
                                 // finite multipliers: fma(0, t, x) == x for
                                 // untouched rows and fma(L, 0, x) == x for a
                                 // zero pivot-row entry
-                                pn[k * PR + row] = (finite || (t != T(0) && ev[q])) ? fma(Lv[q], t, xv) : xv;
+                                pn[k * PS + row] = (finite || (t != T(0) && ev[q])) ? fma(Lv[q], t, xv) : xv;
                             }
                         });
                     }
@@ -612,7 +617,7 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
                     if (row < PR)
                         static_for<0, NB>([&](auto kc) {
                             constexpr int k = kc;
-                            x[q][k] = pn[k * PR + row];
+                            x[q][k] = pn[k * PS + row];
                         });
                 });
                 wave_sync();
@@ -628,7 +633,7 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
                 if (64 * (q + 1) <= PR || row < PR) {
                     static_for<0, NB>([&](auto kc) {
                         constexpr int k = kc;
-                        pn[k * PR + pos[q]] = x[q][k];
+                        pn[k * PS + pos[q]] = x[q][k];
                     });
                     S.perm[p & 1][pos[q]] = row;
                     const int sl = (row < NB) ? row : ((pos[q] != row) ? NB + pos[q] : -1);
@@ -669,7 +674,7 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
             });
             T an[4];
 #pragma unroll
-            for (int kk = 0; kk < 4; kk++) an[kk] = -pn[(4 * kk + g) * PR + rowbase + jj];
+            for (int kk = 0; kk < 4; kk++) an[kk] = -pn[(4 * kk + g) * PS + rowbase + jj];
             // the contents that rows moved by panel q's interchanges take in
             // slots 2..CB-1 (slot 1 took them in step 3c)
 #pragma unroll
@@ -733,7 +738,7 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
                         constexpr int q = qc;
                         const int i = l + 64 * q;
                         const bool h = 64 * (q + 1) <= PR || i < PR;
-                        lm[c][q] = pn[c * PR + (h ? i : 0)];
+                        lm[c][q] = pn[c * PS + (h ? i : 0)];
                     });
                 });
                 static_for<0, NB>([&](auto cc) {
@@ -777,7 +782,7 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
                     const int col_off = e >> 4;
                     const int col = j0 + col_off;
                     T v = T(0);
-                    if (col >= r && col <= ue) v = (col_off < NB) ? pn[col_off * PR + c] : S.u12[c * W + (col_off - NB)];
+                    if (col >= r && col <= ue) v = (col_off < NB) ? pn[col_off * PS + c] : S.u12[c * W + (col_off - NB)];
                     dst[e] = v;
                 }
             } else if (lb >= 2) {
@@ -788,7 +793,7 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
                     const int col_off = e >> 4;
                     const int col = j0 + col_off;
                     if (col >= r && col <= ue) {
-                        const T v = (col_off < NB) ? pn[col_off * PR + c] : S.u12[c * W + (col_off - NB)];
+                        const T v = (col_off < NB) ? pn[col_off * PS + c] : S.u12[c * W + (col_off - NB)];
                         ab[(size_t)col * ldab + kv + r - col] = v;
                     }
                 }
@@ -825,7 +830,7 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
                         if constexpr (sizeof(T) == 8) {
                             static_for<(i + 1) / E, NB / E>([&](auto vc) {
                                 constexpr int cv = E * vc;
-                                const V16 t = *reinterpret_cast<const V16*>(pn + i * PR + cv);
+                                const V16 t = *reinterpret_cast<const V16*>(pn + i * PS + cv);
                                 static_for<0, E>([&](auto ec) {
                                     constexpr int e = ec;
                                     constexpr int c = cv + e;
@@ -835,7 +840,7 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
                         } else {
                             static_for<i + 1, NB>([&](auto cc) {
                                 constexpr int c = cc;
-                                lc[i * NB + c] = pn[i * PR + c];
+                                lc[i * NB + c] = pn[i * PS + c];
                             });
                         }
                     });
@@ -981,7 +986,7 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
                     if (temp != T(0))
                         static_for<m + 1, NB>([&](auto cc) {
                             constexpr int c = cc;
-                            u[c] = fma(pn[m * PR + c], temp, u[c]);
+                            u[c] = fma(pn[m * PS + c], temp, u[c]);
                         });
                 });
                 static_for<0, NB>([&](auto cc) {
@@ -1030,14 +1035,14 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
                 if (NB * (p + 1) <= ju) {
                     T an[4];
 #pragma unroll
-                    for (int kk = 0; kk < 4; kk++) an[kk] = -pn[(4 * kk + g) * PR + NB * lb + jj];
+                    for (int kk = 0; kk < 4; kk++) an[kk] = -pn[(4 * kk + g) * PS + NB * lb + jj];
                     A4 c4 = acc[1];
 #pragma unroll
                     for (int kk = 0; kk < 4; kk++) c4 = P::mfma(an[kk], S.u12[(4 * kk + g) * W + jj], c4);
                     acc[1] = c4;
                 }
 #pragma unroll
-                for (int r = 0; r < 4; r++) pnx[jj * PR + NB * (lb - 1) + P::crow(g, r)] = acc[1][r];
+                for (int r = 0; r < 4; r++) pnx[jj * PS + NB * (lb - 1) + P::crow(g, r)] = acc[1][r];
             }
         } else {
             const int l = lane_fresh(), g = l >> 4, jj = l & 15;
@@ -1046,16 +1051,16 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
                 // the panel on the columns left of each pivot step, in a copy
                 // of the panel (sb is free until the next panel's row moves)
                 T* const cp = S.sb;
-                for (int i = l; i < PR * NB; i += 64) cp[i] = pn[i];
+                for (int i = l; i < PS * NB; i += 64) cp[i] = pn[i];
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 for (int c = nc - 1; c >= 0; c--) {
                     const int jp = S.jp[c];
                     if (jp != 0 && l < c) {
-                        const T t = cp[l * PR + c];
-                        cp[l * PR + c] = cp[l * PR + c + jp];
-                        cp[l * PR + c + jp] = t;
+                        const T t = cp[l * PS + c];
+                        cp[l * PS + c] = cp[l * PS + c + jp];
+                        cp[l * PS + c + jp] = t;
                     }
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                     __builtin_amdgcn_wave_barrier();
@@ -1064,12 +1069,12 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
                 for (int c = 0; c < nc; c++) {
                     const int km = min(kl, n - 1 - (j0 + c));
                     for (int i = l; i < kl; i += 64)
-                        ab[(size_t)(j0 + c) * ldab + kv + 1 + i] = (i < km) ? cp[c * PR + c + 1 + i] : T(0);
+                        ab[(size_t)(j0 + c) * ldab + kv + 1 + i] = (i < km) ? cp[c * PS + c + 1 + i] : T(0);
                 }
             }
             if (p + 1 < npan) {
 #pragma unroll
-                for (int r = 0; r < 4; r++) pnx[jj * PR + NB * (RB - 1) + P::crow(g, r)] = acc[0][r];
+                for (int r = 0; r < 4; r++) pnx[jj * PS + NB * (RB - 1) + P::crow(g, r)] = acc[0][r];
             }
             // This wavefront factors the next panel: its global stores are
             // waited for here, where it has time to spare, rather than at the
