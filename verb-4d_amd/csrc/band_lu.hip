@@ -252,16 +252,18 @@ template <typename T, int RB, int CB>
 struct Lds {
     static constexpr int W = NB * (CB - 1);  // window columns right of the panel
     static constexpr int PR = NB * RB;       // window rows
-    // column stride of the panel buffers: 4 doubles of padding put the 16
-    // columns that one 16x16 tile writes on different LDS banks (and keep
-    // every column start 16-byte aligned)
-    static constexpr int PS = PR + 4;
+    // column stride of the panel buffers: 6 elements of padding spread both
+    // the 16 columns that one 16x16 tile writes and the 4 columns that an
+    // MFMA operand reads over the LDS banks (FP64 column starts stay 16-byte
+    // aligned for the L11 reads)
+    static constexpr int PS = PR + 6;
+    static constexpr int LS = NB + 2;        // row stride of linv (bank spread)
     T pnl[2][NB * PS];                       // panels p and p+1, column-major (full-row interchanges)
     T u12[NB * W];
     T sb[2 * NB * W];                        // pre-interchange contents of the rows that move
     T stash[CB * 4 * 64];                    // wavefront 0's accumulators while it factors
     T rw[PR];                                // right-hand-side window (index mod PR)
-    T linv[NB * NB];                         // L11^-1 of the current panel, row-major
+    T linv[NB * LS];                         // L11^-1 of the current panel, row-major
     int slot[PR];                            // sb row holding window row i's old contents, -1: none
     int dslot[PR];                           // sb row whose contents window row i takes, -1: unchanged
     int perm[2][PR];                         // per panel parity: window row that the interchanges put at each position
@@ -869,7 +871,7 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
                 eliminate(I7(), IE());
                 static_for<0, NB>([&](auto cc) {
                     constexpr int c = cc;
-                    S.linv[c * NB + l] = y[c];
+                    S.linv[c * L::LS + l] = y[c];
                 });
             }
 #pragma unroll
@@ -939,7 +941,7 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
             int dsl[4];
 #pragma unroll
             for (int kk = 0; kk < 4; kk++) {
-                la[kk] = S.linv[jj * NB + 4 * kk + g];
+                la[kk] = S.linv[jj * L::LS + 4 * kk + g];
                 dsl[kk] = (4 * kk + g < nc) ? S.dslot[4 * kk + g] : -1;
             }
             // this wavefront's column blocks lb-1, lb-1+(RB-1), ...: their
