@@ -1105,6 +1105,9 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
 constexpr int kBsThreads = 256;
 constexpr int kBsMaxI = 20;  // staged 16-column groups per thread: (kl+ku+16)/16 <= 20
 constexpr int kXRing = 512;  // x entries kept in LDS: >= kl+ku+32, a power of two
+// row stride of the staged diagonal block and of the partial sums: 17 puts
+// the 16 rows that the triangle's lanes read on different LDS banks
+constexpr int kBsPad = NB + 1;
 
 template <typename T, bool BLK, int NI, int D>
 __global__ __launch_bounds__(kBsThreads) void band_backsolve(const T* __restrict__ ab_all,
@@ -1130,7 +1133,7 @@ __global__ __launch_bounds__(kBsThreads) void band_backsolve(const T* __restrict
     T* sh = reinterpret_cast<T*>(sh_raw);
     T* x = sh;               // ring of kXRing: x[i] at i % kXRing
     T* sd = sh + kXRing;     // [NB][NB] diagonal block of the current block row
-    T* part = sd + NB * NB;  // [NB][16] partial sums
+    T* part = sd + NB * kBsPad;  // [NB][16] partial sums (rows padded to kBsPad)
     const int tid = threadIdx.x;
     const int ri = tid & 15, t = tid >> 4;
     const __amdgpu_buffer_rsrc_t rs =
@@ -1173,7 +1176,7 @@ __global__ __launch_bounds__(kBsThreads) void band_backsolve(const T* __restrict
         const int jb = NB * blk;
         {
             const int r = jb + ri;
-            sd[ri * NB + t] = v[0];  // column jb + t
+            sd[ri * kBsPad + t] = v[0];  // column jb + t
             const int hi = (r < n) ? min(r + kv, n - 1) - jb : -1;
             T sacc = T(0);
             static_for<1, NI>([&](auto ic) {
@@ -1182,7 +1185,7 @@ __global__ __launch_bounds__(kBsThreads) void band_backsolve(const T* __restrict
                 const T xv = x[(jb + co) & (kXRing - 1)];
                 sacc = (co <= hi) ? fma(v[i], xv, sacc) : sacc;
             });
-            part[ri * 16 + t] = sacc;
+            part[ri * kBsPad + t] = sacc;
         }
         __syncthreads();
         if (tid < 64) {
@@ -1199,11 +1202,11 @@ __global__ __launch_bounds__(kBsThreads) void band_backsolve(const T* __restrict
             if (live) {
                 T sp = T(0);
 #pragma unroll
-                for (int q = 0; q < 16; q++) sp += part[i * 16 + q];
+                for (int q = 0; q < 16; q++) sp += part[i * kBsPad + q];
                 b = yr - sp;
 #pragma unroll
-                for (int k = 0; k < NB; k++) urow[k] = sd[i * NB + k];
-                rd = T(1) / sd[i * NB + i];
+                for (int k = 0; k < NB; k++) urow[k] = sd[i * kBsPad + k];
+                rd = T(1) / sd[i * kBsPad + i];
             }
 #pragma unroll
             for (int k = NB - 1; k >= 0; k--) {
@@ -1433,7 +1436,7 @@ int launch_backsolve_t(const T* ab, const T* y, T* x, int n, int kl, int ku, int
                        int ublk_cols = 0, size_t ublk_stride = 0) {
     if (kl + ku + NB > NB * kBsMaxI || kl + ku + 2 * NB > kXRing) return VERB_ERR_ARG;
     if (ublk_cols && (ublk_cols > NB * kBsMaxI || ublk_cols < kl + ku + NB)) return VERB_ERR_ARG;
-    const size_t lds = sizeof(T) * ((size_t)kXRing + NB * NB + NB * 16);
+    const size_t lds = sizeof(T) * ((size_t)kXRing + 2 * NB * kBsPad);
     int dev = 0, max_lds = 0;
     hipGetDevice(&dev);
     hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
