@@ -89,7 +89,9 @@ int verb_diffusion_2d(verb_ctx* ctx, double* psd, const double* x, const double*
 
 /* Replaces Diffusion_1D (src/Diffusion_1D.h:69-77): assembly of
  * getCalculationMatrix1DDiffusion (src/Diffusion_1D.cpp:91-184), RHS
- * C + B*f and tridag (src/MatrixSolver.cpp:689-720). */
+ * C + B*f and tridag (src/MatrixSolver.cpp:689-720). Lines of 3..64 points
+ * (the Thomas recurrences are held in registers); longer lines return
+ * VERB_ERR_ARG, which the reference's tridag would accept. */
 int verb_diffusion_1d(verb_ctx* ctx, double* psd, const double* x, int n, double lbc, double ubc, int l_type,
                       int u_type, const double* Dxx, const double* G, const double* sources, const double* losses,
                       double dt);
@@ -101,6 +103,29 @@ int verb_convection_2d(verb_ctx* ctx, double* psd, const double* P, const double
                        int Pl_type, int Pu_type, int Rl_type, int Ru_type,
                        const double* VP, const double* VR, const double* sources, const double* losses,
                        const double* G_conv, double dt);
+
+/* The reference's two compile-time convection options, per call:
+ *   psd_lost (NULL: off): SAVE_PSD_LOST_CONV (src/Convection_2D.h:84-96,
+ *     src/Convection_2D.cpp:150-155). Every outer sub-step adds
+ *     (PSD - PSD*exp(L*dt_sub)) to it, the PSD the exp losses took; it is
+ *     read and written, the caller zeroes it once per time step as the
+ *     reference's loop does (src/VERB4D_Solver.cpp:626-628). Same layout as
+ *     psd: [nP][nR] for the per-call entry, [P][R][V][K] for the batched one.
+ *   fast_convection != 0: FAST_CONVECTION (src/Convection_2D.cpp:80-84), the
+ *     smaller of the MLT and R outer sub-step counts instead of the larger. */
+typedef struct {
+    double* psd_lost;
+    int fast_convection;
+} verb_convection_options;
+
+/* Convection_2D with the options above (opt == NULL: verb_convection_2d);
+ * with psd_lost it is the SAVE_PSD_LOST_CONV signature
+ * Convection_2D(PSD_PR, PSD_lost_PR, ...) (src/Convection_2D.h:84-96). */
+int verb_convection_2d_ex(verb_ctx* ctx, double* psd, const double* P, const double* R, int nP, int nR,
+                          const double* Pl, const double* Pu, const double* Rl, const double* Ru,
+                          int Pl_type, int Pu_type, int Rl_type, int Ru_type,
+                          const double* VP, const double* VR, const double* sources, const double* losses,
+                          const double* G_conv, double dt, const verb_convection_options* opt);
 
 /* Replaces Convection_1D_ULTIMATE_QUICKEST6
  * (src/Convection_1D_ULTIMATE_QUICKEST6.h:71-79). */
@@ -116,7 +141,7 @@ int verb_dgbtrs(verb_ctx* ctx, int64_t n, int64_t kl, int64_t ku, const double* 
                 const int64_t* ipiv, double* b);
 
 /* Replaces tridag (src/MatrixSolver.cpp:689-720) for `lines` independent
- * systems stored line-major (line l occupies [l*n, (l+1)*n)). */
+ * systems stored line-major (line l occupies [l*n, (l+1)*n)); n <= 64. */
 int verb_tridag_batched(verb_ctx* ctx, const double* a, const double* b, const double* c, const double* r,
                         double* u, int n, int lines);
 
@@ -148,7 +173,11 @@ typedef struct verb_grid {
 
 /* Convection_2D over every (V,K) plane (src/VERB4D_Solver.cpp:860-958). */
 int verb_convection_batched(verb_ctx* ctx, const verb_grid* g, double* psd);
-/* Diffusion_1D over every (P,V,K) line (src/VERB4D_Solver.cpp:999-1055). */
+/* The same with verb_convection_options (psd_lost a device [P][R][V][K]
+ * array, PSD_lost_conv of src/VERB4D_Solver.cpp:335-337, 921-948). */
+int verb_convection_batched_ex(verb_ctx* ctx, const verb_grid* g, double* psd, const verb_convection_options* opt);
+/* Diffusion_1D over every (P,V,K) line (src/VERB4D_Solver.cpp:999-1055);
+ * nR <= 64. */
 int verb_radial_diffusion_batched(verb_ctx* ctx, const verb_grid* g, double* psd);
 /* Diffusion_2D over every (P,R) node (src/VERB4D_Solver.cpp:1100-1210).
  * lu_cache/ipiv_cache: NULL for the plain path (factor every call), or
@@ -157,6 +186,20 @@ int verb_radial_diffusion_batched(verb_ctx* ctx, const verb_grid* g, double* psd
 int verb_local_diffusion_batched(verb_ctx* ctx, const verb_grid* g, double* psd, double* lu_cache,
                                  int64_t* ipiv_cache, int recompute_lu);
 size_t verb_lu_cache_doubles(const verb_grid* g);
+
+/* The operator Diffusion_2D assembles for every (P,R) node
+ * (src/Diffusion_2D.cpp:43-130: AddBoundaries_2D on edge points and the
+ * twelve SecondDerivativeApproximation_2D terms inside,
+ * src/MatrixSolver.cpp:118-494) for one sub-step of dt / max(1,
+ * int(dt/sub_dt)), as the batched local diffusion factors it. Device
+ * outputs, energy-fast unknowns i = iK*nV + iV (src/Matrix.cpp:6479-6481):
+ *   diag [nodes][9][n]: the CalculationMatrix diagonals in key order,
+ *     offsets -nV-1, -nV, -nV+1, -1, 0, 1, nV-1, nV, nV+1, with
+ *     diag[s][i] = A(i, i + offset_s) (src/Matrix.cpp:6423-6474);
+ *   b0, c0 [nodes][n]: the diagonals of B and C.
+ * Nodes skipped by include_boundary are not written. */
+int verb_local_assembly(verb_ctx* ctx, const verb_grid* g, const double* psd, double* diag, double* b0,
+                        double* c0);
 size_t verb_lu_cache_ints(const verb_grid* g);
 
 /* Local diffusion with an uneven active pitch-angle extent per node, as the
@@ -187,14 +230,18 @@ int verb_step(verb_ctx* ctx, const verb_grid* g, double* psd, int run_convection
  *     convection and is the V-lower boundary of this step's local diffusion
  *     (:965-966, Vl_BC = PSD.ySlice(0)); the caller keeps it as g->Vl for
  *     later steps as the reference keeps Vl_BC until its next update;
- *   negative_points (host pointer, or NULL): with positive_psd == 0, the
- *     number of PSD values below 0 after the step (:1223-1238); reading it
- *     synchronises the stream.
+ *   negative_points (host pointer, or NULL): the number of PSD values below
+ *     0 after the step (:1223-1238), 0 with positive_psd (:1218-1221);
+ *     reading it synchronises the stream;
+ *   psd_lost, fast_convection: the convection options of
+ *     verb_convection_options.
  * opt == NULL is verb_step. */
 typedef struct {
     int positive_psd;
     double* vl_from_convection;
     long long* negative_points;
+    double* psd_lost;     /* convection's SAVE_PSD_LOST_CONV accumulator (NULL: off) */
+    int fast_convection;  /* FAST_CONVECTION outer sub-steps */
 } verb_step_options;
 int verb_step_ex(verb_ctx* ctx, const verb_grid* g, double* psd, int run_convection, int run_radial,
                  int run_local, double* lu_cache, int64_t* ipiv_cache, int recompute_lu,
@@ -216,10 +263,13 @@ int verb_check_status(verb_ctx* ctx);
  * iterate is the incoming PSD. stats (nullable) synchronises the stream.
  * ------------------------------------------------------------------------ */
 typedef struct {
-    double tol;       /* FP64 residual tolerance, relative to max|b| */
-    int restart;      /* Krylov vectors per FP32 cycle, 1..24 */
-    int max_cycles;   /* cycles before a node falls back to the band LU */
-    double inner_tol; /* residual reduction that ends an FP32 cycle early */
+    double tol;         /* FP64 residual tolerance, relative to max|b| */
+    int restart;        /* Krylov vectors per FP32 cycle, 1..24 */
+    int max_cycles;     /* cycles before a node falls back to the band LU */
+    double inner_tol;   /* residual reduction that ends an FP32 cycle early */
+    int max_iterations; /* inner iterations per node over all cycles before
+                           it falls back (0: restart * max_cycles); the
+                           warm-started path uses 8 */
 } verb_gmres_params;
 
 typedef struct {
@@ -232,6 +282,10 @@ typedef struct {
 
 int verb_local_diffusion_gmres(verb_ctx* ctx, const verb_grid* g, double* psd, const verb_gmres_params* params,
                                verb_gmres_stats* stats);
+/* The same solve from the initial iterate x0 (device, [P][R][V][K] like
+ * psd) instead of the incoming PSD; x0 is only read. */
+int verb_local_diffusion_gmres_x0(verb_ctx* ctx, const verb_grid* g, double* psd, const double* x0,
+                                  const verb_gmres_params* params, verb_gmres_stats* stats);
 /* convection + radial + verb_local_diffusion_gmres */
 int verb_step_gmres(verb_ctx* ctx, const verb_grid* g, double* psd, const verb_gmres_params* params,
                     verb_gmres_stats* stats);

@@ -52,6 +52,10 @@ REF_VARIANTS = {
     "ref": REF_LIB,
     "hipops": os.path.join(HERE, "_ref", "libverbref_hipops.so"),
     "hiplapack": os.path.join(HERE, "_ref", "libverbref_hiplapack.so"),
+    # the reference built with its compile-time options on this path
+    "lucache": os.path.join(HERE, "_ref", "libverbref_lucache.so"),
+    "lost": os.path.join(HERE, "_ref", "libverbref_lost.so"),
+    "fast": os.path.join(HERE, "_ref", "libverbref_fast.so"),
 }
 
 
@@ -72,6 +76,8 @@ def oracle_lib():
         h.orc_diffusion_1d.argtypes = [_dp, _dp, _I, _D, _D, _I, _I, _dp, _dp, _dp, _dp, _D]
         h.orc_quickest6.argtypes = [_dp, _dp, _I, _D, _D, _I, _I, _dp, _D]
         h.orc_convection_2d.argtypes = [_dp, _dp, _dp, _I, _I, _dp, _dp, _dp, _dp, _I, _I, _I, _I] + [_dp] * 5 + [_D]
+        h.orc_convection_2d_ex.argtypes = [_dp, _dp, _dp, _dp, _I, _I, _dp, _dp, _dp, _dp, _I, _I, _I, _I] + [
+            _dp] * 5 + [_D, _I]
         h.orc_step.argtypes = [ctypes.POINTER(Grid), _dp, _I, _I, _I, _I]
         h.orc_grid_struct_size.restype = ctypes.c_size_t
         assert h.orc_grid_struct_size() == ctypes.sizeof(Grid)
@@ -89,15 +95,27 @@ def ref_lib(variant="ref"):
         if not os.path.exists(path):
             raise RuntimeError("reference harness not built (%s)" % path)
         h = ctypes.CDLL(path)
-        h.ref_diffusion_2d.argtypes = [_dp, _dp, _dp, _I, _I, _dp, _dp, _dp, _dp, _I, _I, _I, _I] + [_dp] * 7 + [_D, _D]
-        h.ref_assemble_2d.argtypes = [_I, _I, _dp, _dp, _dp, _dp, _dp, _dp, _I, _I, _I, _I] + [_dp] * 7 + [
-            _D, _dp, _dp, _dp]
-        h.ref_diffusion_1d.argtypes = [_dp, _dp, _I, _D, _D, _I, _I, _dp, _dp, _dp, _dp, _D]
-        h.ref_tridag.argtypes = [_dp, _dp, _dp, _dp, _dp, _I]
-        h.ref_quickest6.argtypes = [_dp, _dp, _I, _D, _D, _I, _I, _dp, _D]
-        h.ref_convection_2d.argtypes = [_dp, _dp, _dp, _I, _I, _dp, _dp, _dp, _dp, _I, _I, _I, _I] + [_dp] * 5 + [_D]
-        h.ref_step_sample.argtypes = [ctypes.POINTER(Grid), _dp, _I, _I, _I, _I, ctypes.c_long, ctypes.c_long,
-                                      ctypes.c_long, ctypes.c_long, ctypes.c_long, ctypes.c_long]
+        # each build exports the entries its options allow
+        sigs = {
+            "ref_diffusion_2d": [_dp, _dp, _dp, _I, _I, _dp, _dp, _dp, _dp, _I, _I, _I, _I] + [_dp] * 7 + [_D, _D],
+            "ref_diffusion_2d_cached": [_dp, _dp, _dp, _I, _I, _dp, _dp, _dp, _dp, _I, _I, _I, _I] + [_dp] * 7 + [
+                _D, _D, _dp, _i64p, _I],
+            "ref_assemble_2d": [_I, _I, _dp, _dp, _dp, _dp, _dp, _dp, _I, _I, _I, _I] + [_dp] * 7 + [
+                _D, _dp, _dp, _dp],
+            "ref_diffusion_1d": [_dp, _dp, _I, _D, _D, _I, _I, _dp, _dp, _dp, _dp, _D],
+            "ref_tridag": [_dp, _dp, _dp, _dp, _dp, _I],
+            "ref_quickest6": [_dp, _dp, _I, _D, _D, _I, _I, _dp, _D],
+            "ref_convection_2d": [_dp, _dp, _dp, _I, _I, _dp, _dp, _dp, _dp, _I, _I, _I, _I] + [_dp] * 5 + [_D],
+            "ref_convection_2d_lost": [_dp, _dp, _dp, _dp, _I, _I, _dp, _dp, _dp, _dp, _I, _I, _I, _I] + [
+                _dp] * 5 + [_D],
+            "ref_step_sample": [ctypes.POINTER(Grid), _dp, _I, _I, _I, _I, ctypes.c_long, ctypes.c_long,
+                                ctypes.c_long, ctypes.c_long, ctypes.c_long, ctypes.c_long],
+            "ref_step_local_cached": [ctypes.POINTER(Grid), _dp, _I, _dp, _i64p, _I, ctypes.c_long,
+                                      ctypes.c_long],
+        }
+        for name, args in sigs.items():
+            if hasattr(h, name):
+                getattr(h, name).argtypes = args
         h.ref_grid_struct_size.restype = ctypes.c_size_t
         assert h.ref_grid_struct_size() == ctypes.sizeof(Grid)
         _ref[variant] = h
@@ -220,6 +238,33 @@ def convection_2d(psd, P, R, nP, nR, Pl, Pu, Rl, Ru, types, VP, VR, S, L, G, dt,
     if st != 0:
         raise RuntimeError("convection_2d status %d" % st)
     return out
+
+
+def convection_2d_ex(psd, lost, P, R, nP, nR, Pl, Pu, Rl, Ru, types, VP, VR, S, L, G, dt, fast=False,
+                     lib="oracle"):
+    """Convection_2D with the reference's build options: lost (None: off)
+    is the SAVE_PSD_LOST_CONV accumulator, fast the FAST_CONVECTION rule.
+    lib "oracle" is the restatement; "lost" / "fast" are the reference built
+    with -DSAVE_PSD_LOST_CONV / -DFAST_CONVECTION. Returns (psd, lost)."""
+    out = _c(psd).copy()
+    lo = None if lost is None else _c(lost).copy()
+    a = [_c(v) for v in (P, R, Pl, Pu, Rl, Ru, VP, VR, S, L, G)]
+    if lib == "oracle":
+        st = oracle_lib().orc_convection_2d_ex(_p(out), _p(lo) if lo is not None else ctypes.cast(None, _dp),
+                                               _p(a[0]), _p(a[1]), nP, nR, _p(a[2]), _p(a[3]), _p(a[4]), _p(a[5]),
+                                               *types, *[_p(v) for v in a[6:]], dt, 1 if fast else 0)
+    elif lib == "lost":
+        assert lo is not None and not fast
+        st = ref_lib("lost").ref_convection_2d_lost(_p(out), _p(lo), _p(a[0]), _p(a[1]), nP, nR, _p(a[2]),
+                                                    _p(a[3]), _p(a[4]), _p(a[5]), *types, *[_p(v) for v in a[6:]],
+                                                    dt)
+    else:
+        assert lib == "fast" and lo is None
+        st = ref_lib("fast").ref_convection_2d(_p(out), _p(a[0]), _p(a[1]), nP, nR, _p(a[2]), _p(a[3]), _p(a[4]),
+                                               _p(a[5]), *types, *[_p(v) for v in a[6:]], dt)
+    if st != 0:
+        raise RuntimeError("convection_2d status %d" % st)
+    return out, lo
 
 
 def grid_struct(w):

@@ -44,6 +44,7 @@ Matrix2D<double> m2(const double* p, size_t n1, size_t n2) {
 
 extern "C" {
 
+#ifndef LU_CACHING
 // Diffusion_2D (src/Diffusion_2D.h:86-111, non-LU_CACHING variant). 2-D
 // arrays are [nx][ny] row-major like Matrix2D.
 int ref_diffusion_2d(double* psd, const double* x, const double* y, int nx, int ny,
@@ -58,6 +59,25 @@ int ref_diffusion_2d(double* psd, const double* x, const double* y, int nx, int 
     std::memcpy(psd, f.plane_array, sizeof(double) * nx * ny);
     return 0;
 }
+#else
+// Diffusion_2D of an LU_CACHING build (src/Diffusion_2D.h:86-99): the band
+// factors are written to lu_cache/ipiv_cache when recompute != 0 and reused
+// otherwise (src/MatrixSolver.cpp:249-277).
+int ref_diffusion_2d_cached(double* psd, const double* x, const double* y, int nx, int ny,
+                            const double* xl, const double* xu, const double* yl, const double* yu,
+                            int xlt, int xut, int ylt, int yut,
+                            const double* Dxx, const double* Dyy, const double* Dxy, const double* Dyx,
+                            const double* G, const double* S, const double* L, double dt, double sub_dt,
+                            double* lu_cache, int64_t* ipiv_cache, int recompute) {
+    Matrix2D<double> f = m2(psd, nx, ny);
+    Diffusion_2D(f, m2(x, nx, ny), m2(y, nx, ny), nx, ny, m1(xl, ny), m1(xu, ny), m1(yl, nx), m1(yu, nx),
+                 bct(xlt), bct(xut), bct(ylt), bct(yut), m2(Dxx, nx, ny), m2(Dyy, nx, ny), m2(Dxy, nx, ny),
+                 m2(Dyx, nx, ny), m2(G, nx, ny), m2(S, nx, ny), m2(L, nx, ny), dt, sub_dt, lu_cache,
+                 reinterpret_cast<long*>(ipiv_cache), recompute != 0);
+    std::memcpy(psd, f.plane_array, sizeof(double) * nx * ny);
+    return 0;
+}
+#endif
 
 // The 9 diagonals (slot (dy+1)*3+(dx+1), offset dy*nx+dx) plus B0, C0 that
 // Diffusion_2D assembles, obtained by running the reference assembly code
@@ -137,6 +157,22 @@ int ref_quickest6(double* psd, const double* x, int n, double lbc, double ubc, i
     return 0;
 }
 
+#ifdef SAVE_PSD_LOST_CONV
+// Convection_2D of a SAVE_PSD_LOST_CONV build (src/Convection_2D.h:84-96):
+// lost [nP][nR] is read and accumulated.
+int ref_convection_2d_lost(double* psd, double* lost, const double* Pg, const double* Rg, int nP, int nR,
+                           const double* Pl, const double* Pu, const double* Rl, const double* Ru, int plt, int put,
+                           int rlt, int rut, const double* VP, const double* VR, const double* S, const double* L,
+                           const double* G, double dt) {
+    Matrix2D<double> f = m2(psd, nP, nR), lo = m2(lost, nP, nR);
+    Convection_2D(f, lo, m2(Pg, nP, nR), m2(Rg, nP, nR), nP, nR, m1(Pl, nR), m1(Pu, nR), m1(Rl, nP), m1(Ru, nP),
+                  bct(plt), bct(put), bct(rlt), bct(rut), m2(VP, nP, nR), m2(VR, nP, nR), m2(S, nP, nR),
+                  m2(L, nP, nR), m2(G, nP, nR), dt);
+    std::memcpy(psd, f.plane_array, sizeof(double) * nP * nR);
+    std::memcpy(lost, lo.plane_array, sizeof(double) * nP * nR);
+    return 0;
+}
+#else
 int ref_convection_2d(double* psd, const double* Pg, const double* Rg, int nP, int nR, const double* Pl,
                       const double* Pu, const double* Rl, const double* Ru, int plt, int put, int rlt, int rut,
                       const double* VP, const double* VR, const double* S, const double* L, const double* G,
@@ -148,6 +184,7 @@ int ref_convection_2d(double* psd, const double* Pg, const double* Rg, int nP, i
     std::memcpy(psd, f.plane_array, sizeof(double) * nP * nR);
     return 0;
 }
+#endif
 
 // Full-grid arrays handed over by pointer; the layout of the struct is the
 // one the oracle uses (oracle/verb_oracle.c, orc_grid).
@@ -167,6 +204,7 @@ static inline size_t idx4(const ref_grid* g, int p, int r, int v, int k) {
     return (((size_t)p * g->nR + r) * g->nV + v) * g->nK + k;
 }
 
+#if !defined(LU_CACHING) && !defined(SAVE_PSD_LOST_CONV)
 static void conv_plane(const ref_grid* g, double* psd, int iV, int iK) {
     const int nP = g->nP, nR = g->nR;
     Matrix2D<double> f(nP, nR), Pg(nP, nR), Rg(nP, nR), VP(nP, nR), VR(nP, nR), S(nP, nR), L(nP, nR), G(nP, nR);
@@ -246,6 +284,40 @@ int ref_step_sample(const ref_grid* g, double* psd, int nthreads, int run_convec
     }
     return 0;
 }
+
+#endif
+
+#ifdef LU_CACHING
+// Local diffusion of an LU_CACHING build over nodes [n0,n1) (n1 = -1: all),
+// each with its own cache slice lu_all + (node-n0)*(2kl+ku+1)*n, ipiv_all +
+// (node-n0)*n (src/VERB4D_Solver.cpp:340-365, 1100-1210).
+int ref_step_local_cached(const ref_grid* g, double* psd, int nthreads, double* lu_all, int64_t* ipiv_all,
+                          int recompute, long n0, long n1) {
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+    if (n1 < 0) n1 = (long)g->nP * g->nR;
+    const int nV = g->nV, nK = g->nK, n = nV * nK;
+    const size_t ldab = 3 * (size_t)(nV + 1) + 1;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long q = n0; q < n1; q++) {
+        const int iP = (int)(q / g->nR), iR = (int)(q % g->nR);
+        if (!g->include_boundary && (iR == 0 || iR == g->nR - 1)) continue;
+        const size_t o = idx4(g, iP, iR, 0, 0);
+        Matrix2D<double> f = m2(psd + o, nV, nK);
+        const double* Vl = g->Vl + ((size_t)iP * g->nR + iR) * nK;
+        const double* Vu = g->Vu + ((size_t)iP * g->nR + iR) * nK;
+        const double* Kl = g->Kl + ((size_t)iP * g->nR + iR) * nV;
+        const double* Ku = g->Ku + ((size_t)iP * g->nR + iR) * nV;
+        Diffusion_2D(f, m2(g->Vg + o, nV, nK), m2(g->Kg + o, nV, nK), nV, nK, m1(Vl, nK), m1(Vu, nK), m1(Kl, nV),
+                     m1(Ku, nV), bct(g->Vl_t), bct(g->Vu_t), bct(g->Kl_t), bct(g->Ku_t), m2(g->DVV + o, nV, nK),
+                     m2(g->DKK + o, nV, nK), m2(g->DVK + o, nV, nK), m2(g->DVK + o, nV, nK),
+                     m2(g->Gloc + o, nV, nK), m2(g->Src + o, nV, nK) * g->local_sources, m2(g->Lloc + o, nV, nK),
+                     g->dt, g->sub_dt, lu_all + (size_t)(q - n0) * ldab * n,
+                     reinterpret_cast<long*>(ipiv_all + (size_t)(q - n0) * n), recompute != 0);
+        std::memcpy(psd + o, f.plane_array, sizeof(double) * n);
+    }
+    return 0;
+}
+#endif
 
 size_t ref_grid_struct_size(void) { return sizeof(ref_grid); }
 

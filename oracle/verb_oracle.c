@@ -602,12 +602,14 @@ static double maxabs(const double* a, int n) {
     return t;
 }
 
-/* Convection_2D (src/Convection_2D.cpp:36-169). 2-D arrays are [nP][nR]. */
-int orc_convection_2d(double* psd, const double* Pg, const double* Rg, int nP, int nR,
-                      const double* Plbc, const double* Pubc, const double* Rlbc, const double* Rubc,
-                      int plt, int put, int rlt, int rut,
-                      const double* VP, const double* VR, const double* S, const double* L, const double* Gc,
-                      double dt_total) {
+/* Convection_2D (src/Convection_2D.cpp:22-169). 2-D arrays are [nP][nR].
+ * lost (nullable): the SAVE_PSD_LOST_CONV accumulator (:150-155); fast:
+ * FAST_CONVECTION's min of the two sub-step counts (:80-84). */
+int orc_convection_2d_ex(double* psd, double* lost, const double* Pg, const double* Rg, int nP, int nR,
+                         const double* Plbc, const double* Pubc, const double* Rlbc, const double* Rubc,
+                         int plt, int put, int rlt, int rut,
+                         const double* VP, const double* VR, const double* S, const double* L, const double* Gc,
+                         double dt_total, int fast) {
     const double maxCour = 0.25;
     double nsP, nsR;
     const int np = nP * nR;
@@ -621,7 +623,8 @@ int orc_convection_2d(double* psd, const double* Pg, const double* Rg, int nP, i
         double nd = (double)dt_total * maxabs(VR, np) / dR / maxCour;
         nsR = nd <= 1 ? 1 : ceil(nd);
     } else nsR = 1;
-    double num_steps = nsP > nsR ? nsP : nsR; /* std::max */
+    double num_steps = fast ? (nsR < nsP ? nsR : nsP)  /* std::min */
+                            : (nsP < nsR ? nsR : nsP); /* std::max */
     double dt = dt_total / num_steps;
     double* lexp = (double*)malloc(sizeof(double) * np);
     for (int i = 0; i < np; i++) lexp[i] = exp(L[i] * dt);
@@ -658,12 +661,25 @@ int orc_convection_2d(double* psd, const double* Pg, const double* Rg, int nP, i
             }
         }
         if (st) break;
-        for (int i = 0; i < np; i++) psd[i] = psd[i] * lexp[i];
+        for (int i = 0; i < np; i++) {
+            const double before = psd[i];
+            psd[i] = psd[i] * lexp[i];
+            if (lost) lost[i] += (before - psd[i]);
+        }
         for (int iP = 0; iP < nP; iP++)
             for (int iR = 1; iR < nR - 1; iR++) psd[iP * nR + iR] = psd[iP * nR + iR] + S[iP * nR + iR] * dt;
     }
     free(lexp); free(line); free(gx); free(gv); free(gG); free(PG);
     return st;
+}
+
+int orc_convection_2d(double* psd, const double* Pg, const double* Rg, int nP, int nR,
+                      const double* Plbc, const double* Pubc, const double* Rlbc, const double* Rubc,
+                      int plt, int put, int rlt, int rut,
+                      const double* VP, const double* VR, const double* S, const double* L, const double* Gc,
+                      double dt_total) {
+    return orc_convection_2d_ex(psd, 0, Pg, Rg, nP, nR, Plbc, Pubc, Rlbc, Rubc, plt, put, rlt, rut, VP, VR, S, L,
+                                Gc, dt_total, 0);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -3,7 +3,8 @@ reference's own hot-path code (oracle/_ref/libverbref.so, compiled by
 oracle/Makefile from the PRBEM/VERB-4D sources) on small seeded inputs.
 
 Run in a container that holds the reference tree:
-    make -C oracle && python tests/golden/make_golden.py
+    make -C oracle && python tests/golden/make_golden.py [options]
+("options" writes only the fixtures of the reference's build options).
 The fixtures are data only (inputs and the reference's outputs).
 """
 import os
@@ -112,13 +113,89 @@ def conv_plane_case(w, iV, iK):
     return d
 
 
+def conv_option_case(w, iV, iK, lost0, fast):
+    """A convection plane through the reference built with one of its
+    options: SAVE_PSD_LOST_CONV (lost0 is the incoming accumulator) or
+    FAST_CONVECTION."""
+    sl = (slice(None), slice(None), iV, iK)
+    d = dict(psd=w.psd0[sl], P=w.f["Pg"][sl], R=w.f["Rg"][sl], Pl=w.bc["Pl"][:, iV, iK], Pu=w.bc["Pu"][:, iV, iK],
+             Rl=w.bc["Rl"][:, iV, iK], Ru=w.bc["Ru"][:, iV, iK],
+             types=np.array((w.types["Pl"], w.types["Pu"], w.types["Rl"], w.types["Ru"]), dtype=np.int64),
+             VP=w.f["VP"][sl], VR=w.f["VR"][sl], S=w.f["sources"][sl], L=w.f["losses_conv"][sl],
+             G=w.f["G_conv"][sl], dt=np.float64(w.dt), fast=np.int64(1 if fast else 0))
+    nP, nR = d["psd"].shape
+    if lost0 is not None:
+        d["lost_in"] = lost0
+    out, lost = O.convection_2d_ex(d["psd"], lost0, d["P"], d["R"], nP, nR, d["Pl"], d["Pu"], d["Rl"], d["Ru"],
+                                   tuple(int(t) for t in d["types"]), d["VP"], d["VR"], d["S"], d["L"], d["G"],
+                                   float(d["dt"]), fast=fast, lib="fast" if fast else "lost")
+    d["out"] = out
+    if lost is not None:
+        d["lost_out"] = lost
+    return d
+
+
+def lucache_case(w, iP, iR, sub_dt):
+    """One node through the reference built with LU_CACHING: a call that
+    factors into the caller's cache, then a call that reuses it."""
+    o = (iP, iR)
+    t = (w.types["Vl"], w.types["Vu"], w.types["Kl"], w.types["Ku"])
+    d = dict(psd=w.psd0[o], x=w.f["Vg"][o], y=w.f["Kg"][o],
+             xl=w.bc["Vl"][o], xu=w.bc["Vu"][o], yl=w.bc["Kl"][o], yu=w.bc["Ku"][o],
+             types=np.array(t, dtype=np.int64),
+             Dxx=w.f["DVV"][o], Dyy=w.f["DKK"][o], Dxy=w.f["DVK"][o], G=w.f["G_local"][o],
+             S=w.f["sources"][o], L=w.f["losses_local"][o], dt=np.float64(w.dt), sub_dt=np.float64(sub_dt))
+    nx, ny = d["psd"].shape
+    n = nx * ny
+    kl = nx + 1
+    ldab = 3 * kl + 1
+    cache = np.zeros(ldab * n)
+    ipiv = np.zeros(n, dtype=np.int64)
+    h = O.ref_lib("lucache")
+    args = [O._c(d[k]) for k in ("x", "y", "xl", "xu", "yl", "yu", "Dxx", "Dyy", "Dxy", "Dxy", "G", "S", "L")]
+    outs = []
+    f = O._c(d["psd"]).copy()
+    for recompute in (1, 0):
+        st = h.ref_diffusion_2d_cached(O._p(f), O._p(args[0]), O._p(args[1]), nx, ny, *[O._p(a) for a in args[2:6]],
+                                       *[int(v) for v in t], *[O._p(a) for a in args[6:]], float(d["dt"]),
+                                       float(d["sub_dt"]), O._p(cache), ipiv.ctypes.data_as(O._i64p), recompute)
+        assert st == 0
+        outs.append(f.copy())
+        if recompute:
+            d["cache"], d["ipiv"] = cache.copy(), ipiv.copy()
+    d["out1"], d["out2"] = outs
+    return d
+
+
 def save(name, d):
     path = os.path.join(HERE, name)
     np.savez_compressed(path, **{k: np.asarray(v) for k, v in d.items()})
     print("wrote", path, os.path.getsize(path), "bytes")
 
 
+def option_fixtures():
+    """Fixtures of the reference's build options on this path
+    (oracle/_ref/libverbref_{lost,fast,lucache}.so)."""
+    if not all(O.ref_available(v) for v in ("lost", "fast", "lucache")):
+        O.build()
+    w0r = W.config0(seed=2, rich=True)
+    w1r = W.config1(seed=1001, rich=True)
+    w1r.set_kp(6.0)
+    rng = np.random.default_rng(11)
+    save("conv_lost_c0_plane.npz", conv_option_case(w0r, 20, 13, 1e-3 * rng.random((w0r.nP, w0r.nR)), False))
+    save("conv_lost_c1_plane.npz", conv_option_case(w1r, 77, 40, np.zeros((w1r.nP, w1r.nR)), False))
+    save("conv_fast_c0_plane.npz", conv_option_case(w0r, 20, 13, None, True))
+    w1 = W.config1(seed=1001)
+    w1.set_kp(6.0)
+    save("conv_fast_c1_plane.npz", conv_option_case(w1, 77, 40, None, True))
+    ws = W.Workload(nP=2, nR=3, nV=15, nK=13, seed=12, kp=3.0, rich=True)
+    save("local_lucache_node.npz", lucache_case(ws, 1, 1, 100.0))
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "options":
+        option_fixtures()
+        return
     if not O.ref_available():
         O.build()
     # local diffusion: config0 nodes (plain and rich inputs, 2 sub-steps,

@@ -66,3 +66,23 @@ def test_grid_struct_layout_matches_oracle():
     assert ctypes.sizeof(verb4d.VerbGrid) == ctypes.sizeof(O.Grid)
     for (a, ta), (b, tb) in zip(verb4d.VerbGrid._fields_, O.Grid._fields_):
         assert getattr(verb4d.VerbGrid, a).offset == getattr(O.Grid, b).offset
+
+
+@pytest.mark.parametrize("tu,define", [("Convection_2D_hip.cpp", None), ("Convection_2D_hip.cpp", "SAVE_PSD_LOST_CONV"),
+                                       ("Convection_2D_hip.cpp", "FAST_CONVECTION"),
+                                       ("Diffusion_2D_hip.cpp", None), ("Diffusion_2D_hip.cpp", "LU_CACHING")])
+def test_dropins_compile_under_reference_build_options(tu, define):
+    """Each C++ drop-in compiles against the reference's headers with the
+    build option that selects its signature (src/Convection_2D.h:84-109,
+    src/Diffusion_2D.h:86-111)."""
+    import shutil
+    import subprocess
+    ref = "/root/reference/src"
+    if not os.path.isdir(ref) or shutil.which("g++") is None:
+        pytest.skip("reference headers not present")
+    cmd = ["g++", "-std=c++17", "-fsyntax-only", "-w", "-I" + ref, "-I" + os.path.join(ROOT, "include"),
+           "-I" + os.path.join(ROOT, "verb-4d_amd", "dropin"), os.path.join(ROOT, "verb-4d_amd", "dropin", tu)]
+    if define:
+        cmd.insert(1, "-D" + define)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

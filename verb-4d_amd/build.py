@@ -48,16 +48,41 @@ def _run(cmd):
     return r
 
 
-def build(verbose=False):
+def _stale(obj, src):
+    """True when obj is older than its source or any shared header."""
+    if not os.path.exists(obj):
+        return True
+    deps = [src, os.path.join(ROOT, "include", "verb4d_hip.h")]
+    deps += [os.path.join(CSRC, h) for h in os.listdir(CSRC) if h.endswith(".h")]
+    t = os.path.getmtime(obj)
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def build(verbose=False, incremental=False, jobs=8):
+    """Compiles every source (in parallel), then links. incremental=True
+    recompiles only objects older than their source or a header."""
     os.makedirs(OUT_DIR, exist_ok=True)
-    objs = []
+    objs, procs = [], []
     for src, extra in SOURCES:
         obj = os.path.join(OUT_DIR, src.replace(".hip", ".o"))
-        cmd = [HIPCC] + COMMON + extra + ["-c", os.path.join(CSRC, src), "-o", obj]
+        objs.append(obj)
+        path = os.path.join(CSRC, src)
+        if incremental and not _stale(obj, path):
+            continue
+        cmd = [HIPCC] + COMMON + extra + ["-c", path, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
-        _run(cmd)
-        objs.append(obj)
+        while len([p for p in procs if p[0].poll() is None]) >= jobs:
+            procs[0][0].wait()
+        procs.append((subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True), cmd))
+    failed = []
+    for p, cmd in procs:
+        out, err = p.communicate()
+        if p.returncode != 0:
+            sys.stderr.write(" ".join(cmd) + "\n" + out + err)
+            failed.append(cmd[-3])
+    if failed:
+        raise RuntimeError("hipcc failed for %s" % ", ".join(failed))
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)
@@ -65,4 +90,4 @@ def build(verbose=False):
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    print(build(verbose=True, incremental="-i" in sys.argv))

@@ -11,10 +11,6 @@
 #include "common.h"
 #include "gmres.h"
 
-namespace verb {
-int launch_convection_mode(const verb_grid& g, double* psd, int* status, hipStream_t stream, int single_outer,
-                           int tq);
-}
 
 struct verb_ctx {
     int device = 0;
@@ -26,6 +22,9 @@ struct verb_ctx {
     // (uncached) local-diffusion path
     std::vector<std::pair<void*, size_t>> bufs;
     size_t scratch_bytes = 0;
+    // bumped whenever a scratch slot moves or the device-side extent tables
+    // change: launches captured into a graph hold those addresses/contents
+    unsigned long long scratch_gen = 0;
     // kernel-phase timing
     bool timing = false;
     struct Mark { int phase; hipEvent_t a, b; };
@@ -43,6 +42,7 @@ struct verb_ctx {
     double* graph_psd = nullptr;
     int graph_tune[3] = {0, -1, 0};
     std::vector<int> graph_k0;  // pitch-angle extents the graph was captured for (empty: uniform)
+    unsigned long long graph_gen = 0;  // scratch_gen at capture
     // the uneven extents last uploaded (first column, unknowns, dispatch order)
     std::vector<int> uneven_k0;
 };
@@ -69,6 +69,7 @@ void* scratch(verb_ctx* c, int slot, size_t bytes) {
         c->scratch_bytes -= b.second;
         b.first = nullptr;
         b.second = 0;
+        c->scratch_gen++;
         if (hipMalloc(&b.first, bytes) != hipSuccess) return nullptr;
         b.second = bytes;
         c->scratch_bytes += bytes;
@@ -162,6 +163,10 @@ void mark_end(verb_ctx* c, int idx) {
 }
 
 }  // namespace
+
+namespace verb {
+int ctx_error(verb_ctx* c, int code, const char* what) { return set_err(c, code, what); }
+}  // namespace verb
 
 extern "C" {
 
@@ -263,13 +268,18 @@ static int grid_ok(const verb_grid* g) {
     return 1;
 }
 
-int verb_convection_batched(verb_ctx* c, const verb_grid* g, double* psd) {
+int verb_convection_batched_ex(verb_ctx* c, const verb_grid* g, double* psd, const verb_convection_options* opt) {
     if (!c || !grid_ok(g) || !psd) return VERB_ERR_ARG;
     std::lock_guard<std::mutex> lk(c->mu);
     const int m = mark_begin(c, VERB_PHASE_CONVECTION);
-    int rc = verb::launch_convection(*g, psd, c->status, c->stream, c->conv_tq);
+    int rc = verb::launch_convection(*g, psd, c->status, c->stream, c->conv_tq, opt ? opt->psd_lost : nullptr,
+                                     opt ? opt->fast_convection : 0);
     mark_end(c, m);
     return rc ? set_err(c, rc, "convection launch rejected") : VERB_OK;
+}
+
+int verb_convection_batched(verb_ctx* c, const verb_grid* g, double* psd) {
+    return verb_convection_batched_ex(c, g, psd, nullptr);
 }
 
 int verb_radial_diffusion_batched(verb_ctx* c, const verb_grid* g, double* psd) {
@@ -280,6 +290,17 @@ int verb_radial_diffusion_batched(verb_ctx* c, const verb_grid* g, double* psd) 
     int rc = verb::launch_radial(*g, psd, c->status, c->stream);
     mark_end(c, m);
     return rc ? set_err(c, rc, "radial launch rejected") : VERB_OK;
+}
+
+int verb_local_assembly(verb_ctx* c, const verb_grid* g, const double* psd, double* diag, double* b0, double* c0) {
+    if (!c || !grid_ok(g) || !psd || !diag || !b0 || !c0 || !(g->sub_dt > 0)) return VERB_ERR_ARG;
+    if (g->nV < 3 || g->nK < 3) return VERB_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    const size_t n = (size_t)g->nV * g->nK * g->nP * g->nR;
+    double* rhs = (double*)scratch(c, kSlotLocal + 4, n * sizeof(double));
+    if (!rhs) return set_err(c, VERB_ERR_NOMEM, "device allocation failed");
+    int rc = verb::launch_local_assembly(*g, psd, diag, b0, c0, rhs, c->status, c->stream);
+    return rc ? set_err(c, rc, "assembly launch rejected") : VERB_OK;
 }
 
 size_t verb_lu_cache_doubles(const verb_grid* g) {
@@ -344,6 +365,7 @@ int verb_local_diffusion_uneven(verb_ctx* c, const verb_grid* g, double* psd, co
         // the host vectors die at return: wait for the copies
         HIPCHK(c, hipStreamSynchronize(c->stream));
         c->uneven_k0 = hk;
+        c->scratch_gen++;
     }
     LocalBuffers lb = local_buffers(c, g, lu_cache, ipiv_cache, kSlotLocal);
     if (lb.rc) return lb.rc;
@@ -366,6 +388,7 @@ int verb_local_diffusion_uneven(verb_ctx* c, const verb_grid* g, double* psd, co
 
 int verb_step_uneven(verb_ctx* c, const verb_grid* g, double* psd, const int* k0, int run_convection, int run_radial,
                      int run_local) {
+    if (!c || !grid_ok(g) || !psd) return VERB_ERR_ARG;
     int rc = VERB_OK;
     if (run_convection && (3 < g->nP || 3 < g->nR)) rc = verb_convection_batched(c, g, psd);
     if (!rc && run_radial && g->nR >= 3) rc = verb_radial_diffusion_batched(c, g, psd);
@@ -377,11 +400,16 @@ int verb_step_uneven(verb_ctx* c, const verb_grid* g, double* psd, const int* k0
 constexpr int kSlotGmres = 60;  // scratch slots 60..66 for the GMRES workspace
 constexpr int kSlotTune = 70;   // the autotuner's copy of the PSD
 
-int verb_local_diffusion_gmres(verb_ctx* c, const verb_grid* g, double* psd, const verb_gmres_params* prm,
-                               verb_gmres_stats* stats) {
+}  // extern "C"
+
+namespace {
+
+int gmres_local(verb_ctx* c, const verb_grid* g, double* psd, const double* x0, const verb_gmres_params* prm,
+                verb_gmres_stats* stats) {
     if (!c || !grid_ok(g) || !psd || !prm) return VERB_ERR_ARG;
     if (!(g->sub_dt > 0) || (int)(g->dt / g->sub_dt) > 1) return set_err(c, VERB_ERR_ARG, "GMRES path: one sub-step only");
-    if (prm->restart < 1 || prm->restart > verb::kGmMaxM || prm->max_cycles < 0 || !(prm->tol > 0))
+    if (prm->restart < 1 || prm->restart > verb::kGmMaxM || prm->max_cycles < 0 || prm->max_iterations < 0 ||
+        !(prm->tol > 0))
         return set_err(c, VERB_ERR_ARG, "GMRES parameters out of range");
     if (g->nV + 1 > 112) return set_err(c, VERB_ERR_ARG, "GMRES path: fallback LU needs nV <= 111");
     const size_t n = (size_t)g->nV * g->nK;
@@ -405,8 +433,8 @@ int verb_local_diffusion_gmres(verb_ctx* c, const verb_grid* g, double* psd, con
         for (auto& e : ev) e = new_event();
         marks = ev;
     }
-    int rc = verb::launch_local_gmres(*g, psd, lb.w, gw, prm->tol, prm->restart, prm->max_cycles, prm->inner_tol,
-                                      c->status, c->stream, marks);
+    int rc = verb::launch_local_gmres(*g, psd, lb.w, gw, prm->tol, prm->restart, prm->max_cycles,
+                                      prm->max_iterations, prm->inner_tol, x0, c->status, c->stream, marks);
     if (marks) {
         c->marks.push_back({VERB_PHASE_LOCAL_ASSEMBLE, ev[0], ev[1]});
         c->marks.push_back({VERB_PHASE_LOCAL_KRYLOV, ev[2], ev[3]});
@@ -431,6 +459,21 @@ int verb_local_diffusion_gmres(verb_ctx* c, const verb_grid* g, double* psd, con
         *stats = st;
     }
     return VERB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int verb_local_diffusion_gmres(verb_ctx* c, const verb_grid* g, double* psd, const verb_gmres_params* prm,
+                               verb_gmres_stats* stats) {
+    return gmres_local(c, g, psd, nullptr, prm, stats);
+}
+
+int verb_local_diffusion_gmres_x0(verb_ctx* c, const verb_grid* g, double* psd, const double* x0,
+                                  const verb_gmres_params* prm, verb_gmres_stats* stats) {
+    if (!x0) return VERB_ERR_ARG;
+    return gmres_local(c, g, psd, x0, prm, stats);
 }
 
 constexpr int kSlotMixed = 80;  // scratch slots 80..90 for the mixed-precision workspace
@@ -508,6 +551,7 @@ int verb_local_diffusion_mixed(verb_ctx* c, const verb_grid* g, double* psd, con
 
 int verb_step_mixed(verb_ctx* c, const verb_grid* g, double* psd, const verb_mixed_params* prm,
                     verb_mixed_stats* stats) {
+    if (!c || !grid_ok(g) || !psd) return VERB_ERR_ARG;
     int rc = VERB_OK;
     if (3 < g->nP || 3 < g->nR) rc = verb_convection_batched(c, g, psd);
     if (!rc && g->nR >= 3) rc = verb_radial_diffusion_batched(c, g, psd);
@@ -517,6 +561,7 @@ int verb_step_mixed(verb_ctx* c, const verb_grid* g, double* psd, const verb_mix
 
 int verb_step_gmres(verb_ctx* c, const verb_grid* g, double* psd, const verb_gmres_params* prm,
                     verb_gmres_stats* stats) {
+    if (!c || !grid_ok(g) || !psd) return VERB_ERR_ARG;
     int rc = VERB_OK;
     if (3 < g->nP || 3 < g->nR) rc = verb_convection_batched(c, g, psd);
     if (!rc && g->nR >= 3) rc = verb_radial_diffusion_batched(c, g, psd);
@@ -648,9 +693,11 @@ int verb_step_graph(verb_ctx* c, const verb_grid* g, double* psd, const int* k0)
     const int tune[3] = {c->conv_tq, c->lu_variant, c->lu_persistent};
     std::vector<int> key;
     if (k0) key.assign(k0, k0 + g->nP * g->nR);
+    // the captured kernels hold scratch addresses and read the extent tables:
+    // any reallocation or re-upload since the capture forces a new one
     const bool match = c->graph_exec && std::memcmp(&c->graph_grid, g, sizeof(verb_grid)) == 0 &&
                        c->graph_psd == psd && std::memcmp(c->graph_tune, tune, sizeof(tune)) == 0 &&
-                       c->graph_k0 == key;
+                       c->graph_k0 == key && c->graph_gen == c->scratch_gen;
     auto run = [&] {
         return k0 ? verb_step_uneven(c, g, psd, k0, 1, 1, 1) : verb_step(c, g, psd, 1, 1, 1, nullptr, nullptr, 1);
     };
@@ -689,11 +736,13 @@ int verb_step_graph(verb_ctx* c, const verb_grid* g, double* psd, const int* k0)
     c->graph_psd = psd;
     std::memcpy(c->graph_tune, tune, sizeof(tune));
     c->graph_k0 = key;
+    c->graph_gen = c->scratch_gen;
     return VERB_OK;
 }
 
 int verb_step(verb_ctx* c, const verb_grid* g, double* psd, int run_convection, int run_radial, int run_local,
               double* lu_cache, int64_t* ipiv_cache, int recompute_lu) {
+    if (!c || !grid_ok(g) || !psd) return VERB_ERR_ARG;
     int rc = VERB_OK;
     if (run_convection && (3 < g->nP || 3 < g->nR)) rc = verb_convection_batched(c, g, psd);
     if (!rc && run_radial && g->nR >= 3) rc = verb_radial_diffusion_batched(c, g, psd);
@@ -707,14 +756,15 @@ constexpr int kSlotDriver = 95;  // negative-point counter
 int verb_step_ex(verb_ctx* c, const verb_grid* g, double* psd, int run_convection, int run_radial, int run_local,
                  double* lu_cache, int64_t* ipiv_cache, int recompute_lu, const verb_step_options* opt) {
     if (!opt) return verb_step(c, g, psd, run_convection, run_radial, run_local, lu_cache, ipiv_cache, recompute_lu);
-    if (!c || !g || !psd) return VERB_ERR_ARG;
+    if (!c || !grid_ok(g) || !psd) return VERB_ERR_ARG;
     const size_t n = (size_t)g->nP * g->nR * g->nV * g->nK;
     constexpr double kFloor = 1e-21;
     int rc = VERB_OK;
     if (opt->positive_psd) rc = verb::launch_clamp_floor(psd, n, kFloor, c->stream);
     verb_grid gl = *g;
     if (!rc && run_convection && (3 < g->nP || 3 < g->nR)) {
-        rc = verb_convection_batched(c, g, psd);
+        const verb_convection_options co{opt->psd_lost, opt->fast_convection};
+        rc = verb_convection_batched_ex(c, g, psd, &co);
         if (!rc && opt->vl_from_convection && g->Vl_type == VERB_BC_CONSTANT_VALUE) {
             rc = verb::launch_vl_from_psd(*g, psd, opt->vl_from_convection, c->stream);
             gl.Vl = opt->vl_from_convection;
@@ -726,7 +776,12 @@ int verb_step_ex(verb_ctx* c, const verb_grid* g, double* psd, int run_convectio
     if (!rc && run_local && g->nV >= 3 && g->nK >= 3)
         rc = verb_local_diffusion_batched(c, &gl, psd, lu_cache, ipiv_cache, recompute_lu);
     if (rc) return rc;
-    if (opt->positive_psd) return verb::launch_clamp_floor(psd, n, kFloor, c->stream);
+    if (opt->positive_psd) {
+        // every value is >= 1e-21 after the final clamp: no negative points
+        // (src/VERB4D_Solver.cpp:1218-1221)
+        if (opt->negative_points) *opt->negative_points = 0;
+        return verb::launch_clamp_floor(psd, n, kFloor, c->stream);
+    }
     if (opt->negative_points) {
         auto* cnt = (unsigned long long*)scratch(c, kSlotDriver, sizeof(unsigned long long));
         if (!cnt) return set_err(c, VERB_ERR_NOMEM, "device allocation failed");
@@ -852,7 +907,7 @@ static int convection_call(verb_ctx* c, double* psd, const double* P, const doub
                            const double* Pl, const double* Pu, const double* Rl, const double* Ru, int Pl_type,
                            int Pu_type, int Rl_type, int Ru_type, const double* VP, const double* VR,
                            const double* sources, const double* losses, const double* G, double dt,
-                           int single_outer) {
+                           int single_outer, double* lost = nullptr, int fast = 0) {
     std::lock_guard<std::mutex> lk(c->mu);
     const size_t np = (size_t)nP * nR;
     int rc = 0;
@@ -876,13 +931,20 @@ static int convection_call(verb_ctx* c, double* psd, const double* P, const doub
     if (rc) return rc;
     g.Pl_type = Pl_type; g.Pu_type = Pu_type; g.Rl_type = Rl_type; g.Ru_type = Ru_type;
     g.dt = dt;
+    double* dlost = nullptr;
+    if (lost) {
+        dlost = (double*)scratch(c, 12, np * sizeof(double));
+        if (!dlost) return set_err(c, VERB_ERR_NOMEM, "device allocation failed");
+        HIPCHK(c, hipMemcpyAsync(dlost, lost, np * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    }
     if ((rc = reset_status(c))) return rc;
-    rc = verb::launch_convection_mode(g, dpsd, c->status, c->stream, single_outer, 0);
+    rc = verb::launch_convection_mode(g, dpsd, c->status, c->stream, single_outer, 0, dlost, fast);
     if (rc) return set_err(c, rc, "convection launch rejected");
     rc = finish(c);
     reset_status(c);
     if (rc) return rc;
     HIPCHK(c, hipMemcpyAsync(psd, dpsd, np * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (lost) HIPCHK(c, hipMemcpyAsync(lost, dlost, np * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return VERB_OK;
 }
@@ -895,6 +957,17 @@ int verb_convection_2d(verb_ctx* c, double* psd, const double* P, const double* 
     if (!bc_ok(Pl_type) || !bc_ok(Pu_type) || !bc_ok(Rl_type) || !bc_ok(Ru_type)) return VERB_ERR_BC;
     return convection_call(c, psd, P, R, nP, nR, Pl, Pu, Rl, Ru, Pl_type, Pu_type, Rl_type, Ru_type, VP, VR, sources,
                            losses, G_conv, dt, 0);
+}
+
+int verb_convection_2d_ex(verb_ctx* c, double* psd, const double* P, const double* R, int nP, int nR,
+                          const double* Pl, const double* Pu, const double* Rl, const double* Ru, int Pl_type,
+                          int Pu_type, int Rl_type, int Ru_type, const double* VP, const double* VR,
+                          const double* sources, const double* losses, const double* G_conv, double dt,
+                          const verb_convection_options* opt) {
+    if (!c || !psd || nP < 1 || nR < 1) return VERB_ERR_ARG;
+    if (!bc_ok(Pl_type) || !bc_ok(Pu_type) || !bc_ok(Rl_type) || !bc_ok(Ru_type)) return VERB_ERR_BC;
+    return convection_call(c, psd, P, R, nP, nR, Pl, Pu, Rl, Ru, Pl_type, Pu_type, Rl_type, Ru_type, VP, VR, sources,
+                           losses, G_conv, dt, 0, opt ? opt->psd_lost : nullptr, opt ? opt->fast_convection : 0);
 }
 
 int verb_convection_1d_quickest6(verb_ctx* c, double* psd, const double* x, int n, double lbc, double ubc,

@@ -22,10 +22,19 @@ __device__ inline void raise_pivot(int* status, int col1, int node) {
     if (col1 < old) atomicExch(&status[2], node);
 }
 
+// Records `what` as the context's last error and returns code (api.hip).
+int ctx_error(verb_ctx* c, int code, const char* what);
+
 // Launchers (defined in the .hip files). All asynchronous on `stream`.
 int launch_radial(const verb_grid& g, double* psd, int* status, hipStream_t stream);
-// tq: (V,K) planes per workgroup (0: the largest of 4, 2, 1 that fits in LDS)
-int launch_convection(const verb_grid& g, double* psd, int* status, hipStream_t stream, int tq = 0);
+// tq: (V,K) planes per workgroup (0: the largest of 4, 2, 1 that fits in LDS);
+// lost (nullable, [P][R][V][K]): SAVE_PSD_LOST_CONV accumulation; fast:
+// FAST_CONVECTION outer sub-step rule.
+int launch_convection(const verb_grid& g, double* psd, int* status, hipStream_t stream, int tq = 0,
+                      double* lost = nullptr, int fast = 0);
+// single_outer: one outer step (a bare QUICKEST6 line)
+int launch_convection_mode(const verb_grid& g, double* psd, int* status, hipStream_t stream, int single_outer,
+                           int tq, double* lost = nullptr, int fast = 0);
 
 struct LocalWork {
     double* ab;       // band factors, per node (2kl+ku+1)*n doubles (LAPACK layout)
@@ -94,9 +103,11 @@ struct GmresWork {
     int* conv;     // [nodes]
     int* iters;    // [nodes]
 };
+// x0 (nullable, [P][R][V][K] like psd): initial iterate; null starts from psd.
+// max_iters > 0 caps the inner iterations of a node over all its cycles.
 int launch_local_gmres(const verb_grid& g, double* psd, const LocalWork& w, const GmresWork& gw, double tol,
-                       int restart, int max_cycles, double inner_tol, int* status, hipStream_t stream,
-                       hipEvent_t* marks = nullptr);
+                       int restart, int max_cycles, int max_iters, double inner_tol, const double* x0, int* status,
+                       hipStream_t stream, hipEvent_t* marks = nullptr);
 
 // Local diffusion with an FP32 band LU and FP64 iterative refinement:
 // x0 = 0; repeat { r = b - A x (FP64); stop the node when max|r| <= tol*max|b|;
@@ -116,6 +127,11 @@ struct MixedWork {
 };
 int launch_local_mixed(const verb_grid& g, double* psd, const LocalWork& w, const MixedWork& mw, double tol,
                        int max_sweeps, int* status, hipStream_t stream, hipEvent_t* marks = nullptr);
+
+// The assembled local operator of every node (the plain path's assembly):
+// diag [nodes][9][n], b0/c0/rhs [nodes][n], energy-fast; rhs = B0*psd + C0.
+int launch_local_assembly(const verb_grid& g, const double* psd, double* diag, double* b0, double* c0, double* rhs,
+                          int* status, hipStream_t stream);
 
 // Single-system LAPACK band routines on device buffers (one system).
 int launch_dgbtrf(int n, int kl, int ku, double* ab, int ldab, int64_t* ipiv, int* status, hipStream_t stream);

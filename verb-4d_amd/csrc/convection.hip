@@ -381,9 +381,13 @@ __device__ void quickest_step(const verb_grid& g, const Tile& T, int dir, int k,
     __syncthreads();
 }
 
+// lost (nullable, [P][R][V][K]): SAVE_PSD_LOST_CONV, each outer sub-step
+// adds what the exp losses took (Convection_2D.cpp:150-155); fast:
+// FAST_CONVECTION, the smaller of the two outer sub-step counts (:80-84).
 __global__ __launch_bounds__(kThreads) void convection_kernel(const verb_grid g, double* __restrict__ psd,
                                                               int* __restrict__ status, int TQ, int ntiles,
-                                                              int per_xcd, int single_outer) {
+                                                              int per_xcd, int single_outer,
+                                                              double* __restrict__ lost, int fast) {
     // XCD-aware tile order: consecutive workgroup ids land on different
     // XCDs (round-robin), so give each XCD a contiguous run of tiles to keep
     // neighbouring 64-byte plane segments in one L2.
@@ -457,7 +461,8 @@ __global__ __launch_bounds__(kThreads) void convection_kernel(const verb_grid g,
                 nsR = nd <= 1 ? 1 : ceil(nd);
             }
         }
-        double ns = (nsP < nsR) ? nsR : nsP;  // std::max
+        double ns = fast ? ((nsR < nsP) ? nsR : nsP)   // std::min
+                         : ((nsP < nsR) ? nsR : nsP);  // std::max
         if (single_outer) ns = 1;  // a bare QUICKEST6 call: inner sub-steps only
         // A count beyond kMaxSubsteps (or an infinite one) could never finish:
         // report it and leave the plane unchanged; NaN counts run no step, as
@@ -594,7 +599,11 @@ __global__ __launch_bounds__(kThreads) void convection_kernel(const verb_grid g,
             if (!active[tq]) continue;
             const size_t gi = ((size_t)p * g.nR + r) * NQ + T.q0 + tq;
             const double d = dtq[tq];
-            double v = X[t] * exp(g.losses_conv[gi] * d);
+            const double before = X[t];
+            double v = before * exp(g.losses_conv[gi] * d);
+            // the point's thread is the same on every sub-step: no other
+            // lane touches lost[gi]
+            if (lost) lost[gi] = lost[gi] + (before - v);
             if (r >= 1 && r < g.nR - 1) v = v + g.sources[gi] * d;
             X[t] = v;
         }
@@ -623,7 +632,7 @@ size_t convection_lds_bytes(int nP, int nR, int TQ) {
 int convection_tq = 0;  // 0 = choose automatically
 
 int launch_convection_mode(const verb_grid& g, double* psd, int* status, hipStream_t stream, int single_outer,
-                           int tq) {
+                           int tq, double* lost, int fast) {
     if (g.nP < 1 || g.nR < 1) return VERB_ERR_ARG;
     if ((g.nP >= 3 && g.nP < 6) || (g.nR >= 3 && g.nR < 6)) return VERB_ERR_ARG;  // stencil needs 6 points
     if ((g.Pl_type == VERB_BC_PERIODIC) != (g.Pu_type == VERB_BC_PERIODIC)) return VERB_ERR_ARG;
@@ -647,12 +656,13 @@ int launch_convection_mode(const verb_grid& g, double* psd, int* status, hipStre
     const int ntiles = (int)((NQ + TQ - 1) / TQ);
     const int per_xcd = (ntiles + 7) / 8;
     hipLaunchKernelGGL(convection_kernel, dim3(per_xcd * 8), dim3(kThreads), lds, stream, g, psd, status, TQ, ntiles,
-                       per_xcd, single_outer);
+                       per_xcd, single_outer, lost, fast);
     return hipGetLastError() == hipSuccess ? VERB_OK : VERB_ERR_HIP;
 }
 
-int launch_convection(const verb_grid& g, double* psd, int* status, hipStream_t stream, int tq) {
-    return launch_convection_mode(g, psd, status, stream, 0, tq);
+int launch_convection(const verb_grid& g, double* psd, int* status, hipStream_t stream, int tq, double* lost,
+                      int fast) {
+    return launch_convection_mode(g, psd, status, stream, 0, tq, lost, fast);
 }
 
 }  // namespace verb

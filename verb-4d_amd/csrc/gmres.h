@@ -17,6 +17,7 @@ struct GmArgs {
     int* iters;          // [nodes] inner iterations used
     int nx, ny, nfull;
     int m, max_cycles;
+    int max_iters;       // inner iterations per node at most, over all cycles (0: m * max_cycles)
     double tol;          // max|b - A x| <= tol * max|b|
     float inner_tol;     // residual reduction that ends an FP32 cycle early
     int nR, include_boundary;

@@ -10,8 +10,9 @@
 //     the exact inverse of the pitch-angle tridiagonal blocks (for every
 //     energy, the +-nV couplings of A), classical Gram-Schmidt with the
 //     basis vectors kept unnormalised and their scales held in LDS.
-// A node that has not met tol after max_cycles cycles is flagged; the
-// caller factors exactly those nodes with the band LU.
+// A node that has not met tol after max_cycles cycles, or within
+// max_iters inner iterations in all, is flagged; the caller factors exactly
+// those nodes with the band LU.
 #include "common.h"
 #include "gmres.h"
 
@@ -174,6 +175,9 @@ __global__ __launch_bounds__(kGmThreads) void gmres_nodes(GmArgs a) {
             break;
         }
         if (cyc == a.max_cycles) break;
+        // the iteration budget: a node that has used it is left to the caller
+        const int mc = (a.max_iters > 0) ? min(m, a.max_iters - total) : m;
+        if (mc < 1) break;
         block_sum(rs, 1, red, hs);
         const float beta = sqrtf(hs[0]);
         if (!(beta > 0.0f)) break;
@@ -185,7 +189,7 @@ __global__ __launch_bounds__(kGmThreads) void gmres_nodes(GmArgs a) {
         }
         __syncthreads();
 
-        for (int j = 0; j < m; j++) {
+        for (int j = 0; j < mc; j++) {
             // z = M^-1 v_j
             precond(V + (size_t)j * nf, scl[j], z, ps, pc, pd, nx, ny);
             __syncthreads();
@@ -281,7 +285,7 @@ __global__ __launch_bounds__(kGmThreads) void gmres_nodes(GmArgs a) {
 }
 
 int launch_gmres(const GmArgs& a, int nodes, hipStream_t stream) {
-    if (a.m < 1 || a.m > kGmMaxM || a.max_cycles < 0) return VERB_ERR_ARG;
+    if (a.m < 1 || a.m > kGmMaxM || a.max_cycles < 0 || a.max_iters < 0) return VERB_ERR_ARG;
     hipLaunchKernelGGL(gmres_nodes, dim3(nodes), dim3(kGmThreads), 0, stream, a);
     return hipGetLastError() == hipSuccess ? VERB_OK : VERB_ERR_HIP;
 }

@@ -671,6 +671,14 @@ int launch_local(const verb_grid& g, double* psd, const LocalWork& w, bool facto
     return hipGetLastError() == hipSuccess ? VERB_OK : VERB_ERR_HIP;
 }
 
+int launch_local_assembly(const verb_grid& g, const double* psd, double* diag, double* b0, double* c0, double* rhs,
+                          int* status, hipStream_t stream) {
+    if (g.nV < 3 || g.nK < 3) return VERB_ERR_ARG;
+    const int num_substeps = max(1, (int)(g.dt / g.sub_dt));
+    launch_assemble(g, psd, nullptr, b0, c0, rhs, diag, g.dt / num_substeps, 0, 1, status, nullptr, stream);
+    return hipGetLastError() == hipSuccess ? VERB_OK : VERB_ERR_HIP;
+}
+
 int launch_dgbtrf(int n, int kl, int ku, double* ab, int ldab, int64_t* ipiv, int* status, hipStream_t stream) {
     if (n < 1 || kl < 0 || ku < 0 || ldab < 2 * kl + ku + 1) return VERB_ERR_ARG;
     const size_t lds = sizeof(double) * (size_t)(kl + 1 + kl + ku + 1);
@@ -693,8 +701,8 @@ int launch_dgbtrs(int n, int kl, int ku, const double* ab, int ldab, const int64
 
 
 int launch_local_gmres(const verb_grid& g, double* psd, const LocalWork& w, const GmresWork& gw, double tol,
-                       int restart, int max_cycles, double inner_tol, int* status, hipStream_t stream,
-                       hipEvent_t* marks) {
+                       int restart, int max_cycles, int max_iters, double inner_tol, const double* x0, int* status,
+                       hipStream_t stream, hipEvent_t* marks) {
     if (g.nV < 3 || g.nK < 3) return VERB_ERR_ARG;
     const int nx = g.nV, ny = g.nK, n = nx * ny;
     const int kl = nx + 1, ku = nx + 1, ldab = 2 * kl + ku + 1;
@@ -706,7 +714,8 @@ int launch_local_gmres(const verb_grid& g, double* psd, const LocalWork& w, cons
     dim3 agrid((n + kAsmThreads - 1) / kAsmThreads, nodes);
     if (marks) hipEventRecord(marks[0], stream);
     launch_assemble(g, psd, w.ab, w.b0, w.c0, w.rhs, w.diag, dt_sub, 0, 1, status, nullptr, stream);
-    hipLaunchKernelGGL(gather_psd, agrid, dim3(kAsmThreads), 0, stream, g, psd, gw.x);
+    // initial iterate: the caller's x0 (same [P][R][V][K] layout as psd) or the incoming PSD
+    hipLaunchKernelGGL(gather_psd, agrid, dim3(kAsmThreads), 0, stream, g, x0 ? x0 : psd, gw.x);
     if (marks) {
         hipEventRecord(marks[1], stream);
         hipEventRecord(marks[2], stream);
@@ -714,7 +723,7 @@ int launch_local_gmres(const verb_grid& g, double* psd, const LocalWork& w, cons
     GmArgs ga;
     ga.diag = w.diag; ga.rhs = w.rhs; ga.x = gw.x; ga.d32 = gw.d32; ga.pf = gw.pf; ga.V = gw.V; ga.z = gw.z;
     ga.conv = gw.conv; ga.iters = gw.iters;
-    ga.nx = nx; ga.ny = ny; ga.nfull = n; ga.m = restart; ga.max_cycles = max_cycles;
+    ga.nx = nx; ga.ny = ny; ga.nfull = n; ga.m = restart; ga.max_cycles = max_cycles; ga.max_iters = max_iters;
     ga.tol = tol; ga.inner_tol = (float)inner_tol; ga.nR = g.nR; ga.include_boundary = g.include_boundary;
     int rc = launch_gmres(ga, nodes, stream);
     if (rc) return rc;

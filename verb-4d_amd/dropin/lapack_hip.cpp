@@ -1,5 +1,5 @@
 // Drop-in for the LAPACK routines MatrixSolver binds
-// (src/MatrixSolver.h:94-98): dgbtrf_ / dgbtrs_ with the Fortran ABI and
+// (src/MatrixSolver.h:243-248): dgbtrf_ / dgbtrs_ with the Fortran ABI and
 // ILP64 `long` integers the reference declares, computed on the GPU
 // (LAPACK pivoting, dgbtf2 storage convention). Linking this object in place
 // of a LAPACK library moves every band factorisation the reference's own

@@ -49,14 +49,21 @@ class StepOptions(ctypes.Structure):
     """verb_step_options: the time loop's clamp, V-lower boundary from
     convection and negative-point count around the step."""
     _fields_ = [("positive_psd", ctypes.c_int), ("vl_from_convection", ctypes.c_void_p),
-                ("negative_points", ctypes.POINTER(ctypes.c_longlong))]
+                ("negative_points", ctypes.POINTER(ctypes.c_longlong)), ("psd_lost", ctypes.c_void_p),
+                ("fast_convection", ctypes.c_int)]
+
+
+class ConvectionOptions(ctypes.Structure):
+    """verb_convection_options: SAVE_PSD_LOST_CONV accumulator and the
+    FAST_CONVECTION sub-step rule."""
+    _fields_ = [("psd_lost", ctypes.c_void_p), ("fast_convection", ctypes.c_int)]
 
 
 class GmresParams(ctypes.Structure):
     """ctypes mirror of ``verb_gmres_params`` (include/verb4d_hip.h)."""
 
     _fields_ = (("tol", ctypes.c_double), ("restart", ctypes.c_int), ("max_cycles", ctypes.c_int),
-                ("inner_tol", ctypes.c_double))
+                ("inner_tol", ctypes.c_double), ("max_iterations", ctypes.c_int))
 
 
 class GmresStats(ctypes.Structure):
@@ -144,6 +151,9 @@ _SIGNATURES = [
     ("verb_convection_2d", ctypes.c_int,
      [_P, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp] + [ctypes.c_int] * 4
      + [_dp] * 5 + [ctypes.c_double]),
+    ("verb_convection_2d_ex", ctypes.c_int,
+     [_P, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp] + [ctypes.c_int] * 4
+     + [_dp] * 5 + [ctypes.c_double, ctypes.POINTER(ConvectionOptions)]),
     ("verb_convection_1d_quickest6", ctypes.c_int,
      [_P, _dp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp,
       ctypes.c_double]),
@@ -153,6 +163,8 @@ _SIGNATURES = [
      [_P, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _dp, ctypes.c_int64, _i64p, _dp]),
     ("verb_tridag_batched", ctypes.c_int, [_P, _dp, _dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int]),
     ("verb_convection_batched", ctypes.c_int, [_P, ctypes.POINTER(VerbGrid), ctypes.c_void_p]),
+    ("verb_convection_batched_ex", ctypes.c_int,
+     [_P, ctypes.POINTER(VerbGrid), ctypes.c_void_p, ctypes.POINTER(ConvectionOptions)]),
     ("verb_radial_diffusion_batched", ctypes.c_int, [_P, ctypes.POINTER(VerbGrid), ctypes.c_void_p]),
     ("verb_local_diffusion_batched", ctypes.c_int,
      [_P, ctypes.POINTER(VerbGrid), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
@@ -162,6 +174,8 @@ _SIGNATURES = [
     ("verb_step_uneven", ctypes.c_int,
      [_P, ctypes.POINTER(VerbGrid), ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int,
       ctypes.c_int]),
+    ("verb_local_assembly", ctypes.c_int,
+     [_P, ctypes.POINTER(VerbGrid), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ("verb_lu_cache_doubles", ctypes.c_size_t, [ctypes.POINTER(VerbGrid)]),
     ("verb_lu_cache_ints", ctypes.c_size_t, [ctypes.POINTER(VerbGrid)]),
     ("verb_step", ctypes.c_int,
@@ -170,6 +184,9 @@ _SIGNATURES = [
     ("verb_check_status", ctypes.c_int, [_P]),
     ("verb_local_diffusion_gmres", ctypes.c_int,
      [_P, ctypes.POINTER(VerbGrid), ctypes.c_void_p, ctypes.POINTER(GmresParams), ctypes.POINTER(GmresStats)]),
+    ("verb_local_diffusion_gmres_x0", ctypes.c_int,
+     [_P, ctypes.POINTER(VerbGrid), ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(GmresParams),
+      ctypes.POINTER(GmresStats)]),
     ("verb_step_gmres", ctypes.c_int,
      [_P, ctypes.POINTER(VerbGrid), ctypes.c_void_p, ctypes.POINTER(GmresParams), ctypes.POINTER(GmresStats)]),
     ("verb_local_diffusion_mixed", ctypes.c_int,
@@ -339,6 +356,26 @@ def Convection_2D(PSD, P, R, P_size, R_size, P_LBC, P_UBC, R_LBC, R_UBC, P_LBC_t
                                   _ptr(two[3]), _ptr(two[4]), _ptr(two[5]), _ptr(two[6]), float(dt))
     ctx.check(rc)
     return out
+
+
+def Convection_2D_lost(PSD, PSD_lost, P, R, P_size, R_size, P_LBC, P_UBC, R_LBC, R_UBC, P_LBC_type, P_UBC_type,
+                       R_LBC_type, R_UBC_type, VP, VR, Sources, Losses, G_conv, dt, fast_convection=False, ctx=None):
+    """Convection_2D of a SAVE_PSD_LOST_CONV build (src/Convection_2D.h:84-96);
+    returns (PSD, PSD_lost) with PSD_lost accumulated from the given array.
+    fast_convection selects FAST_CONVECTION's sub-step rule."""
+    ctx = ctx or default_context()
+    nP, nR = int(P_size), int(R_size)
+    out = _f64(PSD, (nP, nR)).copy()
+    lost = None if PSD_lost is None else _f64(PSD_lost, (nP, nR)).copy()
+    two = [_f64(v, (nP, nR)) for v in (P, R, VP, VR, Sources, Losses, G_conv)]
+    bc = [_f64(P_LBC, (nR,)), _f64(P_UBC, (nR,)), _f64(R_LBC, (nP,)), _f64(R_UBC, (nP,))]
+    opt = ConvectionOptions(lost.ctypes.data if lost is not None else None, 1 if fast_convection else 0)
+    rc = lib().verb_convection_2d_ex(ctx.handle, _ptr(out), _ptr(two[0]), _ptr(two[1]), nP, nR,
+                                     *[_ptr(b) for b in bc], int(P_LBC_type), int(P_UBC_type), int(R_LBC_type),
+                                     int(R_UBC_type), _ptr(two[2]), _ptr(two[3]), _ptr(two[4]), _ptr(two[5]),
+                                     _ptr(two[6]), float(dt), ctypes.byref(opt))
+    ctx.check(rc)
+    return out, lost
 
 
 def Convection_1D(PSD, x, x_size, x_LBC, x_UBC, x_LBC_type, x_UBC_type, Ux, dt, ctx=None):

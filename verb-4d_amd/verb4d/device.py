@@ -73,19 +73,38 @@ class DeviceGrid:
         self.ctx.check(rc)
 
     def step_ex(self, positive_psd=False, vl_from_convection=None, count_negative=False, run_convection=True,
-                run_radial=True, run_local=True):
+                run_radial=True, run_local=True, psd_lost=None, fast_convection=False):
         """verb_step_ex: the step with the time loop's positivity clamps,
         the V-lower boundary taken from the PSD after convection (into the
-        given device tensor [P][R][K]) and the count of negative points.
-        Returns the count (or None)."""
+        given device tensor [P][R][K]), the count of negative points and the
+        convection options (psd_lost: a device tensor like psd that
+        accumulates the PSD the convection losses take). Returns the count
+        (or None)."""
         cnt = ctypes.c_longlong(-1)
         opt = StepOptions(1 if positive_psd else 0,
                           vl_from_convection.data_ptr() if vl_from_convection is not None else None,
-                          ctypes.pointer(cnt) if count_negative else None)
+                          ctypes.pointer(cnt) if count_negative else None,
+                          psd_lost.data_ptr() if psd_lost is not None else None, 1 if fast_convection else 0)
         rc = lib().verb_step_ex(self.ctx.handle, ctypes.byref(self.grid), self.psd.data_ptr(), int(run_convection),
                                 int(run_radial), int(run_local), None, None, 1, ctypes.byref(opt))
         self.ctx.check(rc)
         return cnt.value if count_negative else None
+
+    def assembly(self):
+        """The assembled local operator of every node (verb_local_assembly):
+        (diag [nodes][9][n], B0 [nodes][n], C0 [nodes][n]) as host arrays,
+        energy-fast unknowns."""
+        torch = self.torch
+        nodes = self.w.nP * self.w.nR
+        n = self.w.nV * self.w.nK
+        diag = torch.zeros((nodes, 9, n), dtype=torch.float64, device=self.device)
+        b0 = torch.zeros((nodes, n), dtype=torch.float64, device=self.device)
+        c0 = torch.zeros((nodes, n), dtype=torch.float64, device=self.device)
+        torch.cuda.synchronize()
+        self.ctx.check(lib().verb_local_assembly(self.ctx.handle, ctypes.byref(self.grid), self.psd.data_ptr(),
+                                                 diag.data_ptr(), b0.data_ptr(), c0.data_ptr()))
+        self.ctx.synchronize()
+        return diag.cpu().numpy(), b0.cpu().numpy(), c0.cpu().numpy()
 
     def convection(self):
         self.ctx.check(lib().verb_convection_batched(self.ctx.handle, ctypes.byref(self.grid), self.psd.data_ptr()))
@@ -116,16 +135,29 @@ class DeviceGrid:
                                               int(run_radial), int(run_local)))
 
     @staticmethod
-    def gmres_params(tol=1e-11, restart=20, max_cycles=8, inner_tol=1e-5):
-        return GmresParams(tol, restart, max_cycles, inner_tol)
+    def gmres_params(tol=1e-11, restart=20, max_cycles=8, inner_tol=1e-5, max_iterations=0):
+        return GmresParams(tol, restart, max_cycles, inner_tol, max_iterations)
 
-    def local_gmres(self, params=None, stats=True):
-        """Local diffusion by matrix-free GMRES with band-LU fallback; returns
+    @staticmethod
+    def gmres_params_capped(tol=1e-10, iterations=8, cycles=2, inner_tol=1e-6):
+        """At most `iterations` inner iterations per node in all (split over
+        `cycles` FP32 cycles, each closed by an FP64 residual), then the band
+        LU for every node that has not met tol."""
+        return GmresParams(tol, (iterations + cycles - 1) // cycles, cycles, inner_tol, iterations)
+
+    def local_gmres(self, params=None, stats=True, x0=None):
+        """Local diffusion by matrix-free GMRES with band-LU fallback, from
+        the incoming PSD or from the device tensor x0 [P][R][V][K]; returns
         the stats dict (synchronises) or None."""
         p = params or self.gmres_params()
         st = GmresStats() if stats else None
-        self.ctx.check(lib().verb_local_diffusion_gmres(self.ctx.handle, ctypes.byref(self.grid), self.psd.data_ptr(),
-                                                        ctypes.byref(p), ctypes.byref(st) if st else None))
+        if x0 is None:
+            rc = lib().verb_local_diffusion_gmres(self.ctx.handle, ctypes.byref(self.grid), self.psd.data_ptr(),
+                                                  ctypes.byref(p), ctypes.byref(st) if st else None)
+        else:
+            rc = lib().verb_local_diffusion_gmres_x0(self.ctx.handle, ctypes.byref(self.grid), self.psd.data_ptr(),
+                                                     x0.data_ptr(), ctypes.byref(p), ctypes.byref(st) if st else None)
+        self.ctx.check(rc)
         return st.as_dict() if st else None
 
     def step_gmres(self, params=None, stats=False):
