@@ -87,6 +87,23 @@ int verb_diffusion_2d(verb_ctx* ctx, double* psd, const double* x, const double*
                       const double* G, const double* sources, const double* losses, double dt, double sub_dt,
                       double* lu_cache, int64_t* ipiv_cache, int recompute_lu);
 
+/* verb_diffusion_2d without any device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments, kernels and results as verb_diffusion_2d;
+ * every device buffer (staged inputs, operator, band factors, pivots,
+ * status words) comes from one static 32 MiB device array per GPU, held
+ * until the process ends. A node needing more (about nx*ny*(3nx+29)*8
+ * bytes; the 101x91 production node takes 23.3 MiB) is rejected with
+ * VERB_ERR_ARG. Host threads of one process are serialised per GPU; calls
+ * that overlap on one GPU otherwise race on the array. */
+int verb_diffusion_2d_noalloc(verb_ctx* ctx, double* psd, const double* x, const double* y, int nx, int ny,
+                              const double* xl, const double* xu, const double* yl, const double* yu,
+                              int xl_type, int xu_type, int yl_type, int yu_type,
+                              const double* Dxx, const double* Dyy, const double* Dxy, const double* Dyx,
+                              const double* G, const double* sources, const double* losses, double dt,
+                              double sub_dt, double* lu_cache, int64_t* ipiv_cache, int recompute_lu);
+
 /* Replaces Diffusion_1D (src/Diffusion_1D.h:69-77): assembly of
  * getCalculationMatrix1DDiffusion (src/Diffusion_1D.cpp:91-184), RHS
  * C + B*f and tridag (src/MatrixSolver.cpp:689-720). Lines of 3..64 points

@@ -26,6 +26,7 @@ SOURCES = [
     ("gmres.hip", []),
     ("driver.hip", []),
     ("api.hip", []),
+    ("noalloc.hip", []),
 ]
 
 COMMON = [

@@ -145,6 +145,9 @@ _SIGNATURES = [
     ("verb_diffusion_2d", ctypes.c_int,
      [_P, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp] + [ctypes.c_int] * 4
      + [_dp] * 7 + [ctypes.c_double, ctypes.c_double, _dp, _i64p, ctypes.c_int]),
+    ("verb_diffusion_2d_noalloc", ctypes.c_int,
+     [_P, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp] + [ctypes.c_int] * 4
+     + [_dp] * 7 + [ctypes.c_double, ctypes.c_double, _dp, _i64p, ctypes.c_int]),
     ("verb_diffusion_1d", ctypes.c_int,
      [_P, _dp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp,
       _dp, ctypes.c_double]),
@@ -301,7 +304,7 @@ def _ptr(a):
 # ---------------------------------------------------------------------------
 def Diffusion_2D(psd, x, y, x_size, y_size, x_LBC, x_UBC, y_LBC, y_UBC, x_LBC_type, x_UBC_type, y_LBC_type,
                  y_UBC_type, Dxx, Dyy, Dxy, Dyx, G, Sources, Losses, dt, sub_dt, lu_cache=None, index_cache=None,
-                 recompute_lu=True, ctx=None):
+                 recompute_lu=True, ctx=None, _entry="verb_diffusion_2d"):
     """Diffusion_2D (src/Diffusion_2D.h:86-111); returns the updated PSD [x][y].
 
     With ``lu_cache``/``index_cache`` (float64 ``(2kl+ku+1)*n`` and int64 ``n``
@@ -323,12 +326,22 @@ def Diffusion_2D(psd, x, y, x_size, y_size, x_LBC, x_UBC, y_LBC, y_UBC, x_LBC_ty
             raise TypeError("lu_cache must be float64 and index_cache int64")
         lu_p = lu_cache.ctypes.data_as(_dp)
         piv_p = index_cache.ctypes.data_as(_i64p)
-    rc = lib().verb_diffusion_2d(
+    rc = getattr(lib(), _entry)(
         ctx.handle, _ptr(out), _ptr(arrs[0]), _ptr(arrs[1]), nx, ny, *[_ptr(b) for b in bcs], int(x_LBC_type),
         int(x_UBC_type), int(y_LBC_type), int(y_UBC_type), *[_ptr(c) for c in coefs], float(dt), float(sub_dt),
         lu_p, piv_p, 1 if recompute_lu else 0)
     ctx.check(rc)
     return out
+
+
+def Diffusion_2D_noalloc(psd, x, y, x_size, y_size, x_LBC, x_UBC, y_LBC, y_UBC, x_LBC_type, x_UBC_type, y_LBC_type,
+                         y_UBC_type, Dxx, Dyy, Dxy, Dyx, G, Sources, Losses, dt, sub_dt, lu_cache=None,
+                         index_cache=None, recompute_lu=True, ctx=None):
+    """Diffusion_2D through verb_diffusion_2d_noalloc (no device allocation:
+    a static per-GPU scratch array); same results as Diffusion_2D."""
+    return Diffusion_2D(psd, x, y, x_size, y_size, x_LBC, x_UBC, y_LBC, y_UBC, x_LBC_type, x_UBC_type, y_LBC_type,
+                        y_UBC_type, Dxx, Dyy, Dxy, Dyx, G, Sources, Losses, dt, sub_dt, lu_cache, index_cache,
+                        recompute_lu, ctx, _entry="verb_diffusion_2d_noalloc")
 
 
 def Diffusion_1D(psd, x, x_size, x_LBC, x_UBC, x_LBC_type, x_UBC_type, Dxx, G, Sources, Losses, dt, ctx=None):
