@@ -84,10 +84,165 @@ def phase_work(w):
     }
 
 
-def cpu_baseline(w, psd, threads, target_s):
-    """The reference CPU path on a bounded sample of the step: each operator
-    on a growing prefix of its slices until it has run target_s seconds (or
-    covered everything), then scaled to one full step."""
+def host_cpu():
+    """(physical cores this process may run on, logical CPUs it may run on,
+    CPUs of the machine, CPU model)."""
+    cpus = sorted(os.sched_getaffinity(0))
+    cores = set()
+    for c in cpus:
+        base = "/sys/devices/system/cpu/cpu%d/topology/" % c
+        try:
+            pkg = open(base + "physical_package_id").read().strip()
+            core = open(base + "core_id").read().strip()
+            cores.add((pkg, core))
+        except OSError:
+            cores.add(("?", str(c)))
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return len(cores), len(cpus), os.cpu_count(), model
+
+
+HOST_CPU = None  # host_cpu() taken before anything else pins this process's threads
+
+
+def cpu_threads(requested):
+    """The baseline's OpenMP threads: the physical cores this process may
+    use, within OMP_NUM_THREADS when the host sets a share."""
+    phys, logical, _, _ = HOST_CPU
+    n = phys
+    share = os.environ.get("OMP_NUM_THREADS")
+    if share and share.isdigit() and int(share) > 0:
+        n = min(n, int(share))
+    if requested > 0:
+        n = min(n, requested)
+    return max(1, n)
+
+
+def node_errors(got, ref, floor=1e-12):
+    """(largest per-(P,R)-node normwise error, largest pointwise relative
+    error over points with |ref| >= floor * the node's max|ref|)."""
+    import numpy as np
+
+    num = np.abs(got - ref)
+    den = np.abs(ref).max(axis=(2, 3), keepdims=True)
+    den = np.where(den == 0, 1.0, den)
+    normwise = float((num.max(axis=(2, 3), keepdims=True) / den).max())
+    mask = np.abs(ref) >= floor * den
+    point = float((num[mask] / np.abs(ref[mask])).max()) if mask.any() else 0.0
+    return normwise, point
+
+
+def cpu_baseline(w, psd, threads, gpu_step, cached_nodes=64):
+    """The reference CPU path (oracle/_ref, the PRBEM/VERB-4D sources built
+    with OpenMP and OpenBLAS) on one full step of the timed workload from the
+    state psd: convection on every (V,K) plane, radial diffusion on every
+    line, local diffusion on every node, each operator timed. Its output is
+    also the check of the GPU step from the same state (gpu_step(psd) ->
+    PSD). With LU caching on, the local step's cached-factor solves are timed
+    on cached_nodes nodes (their factors computed first, untimed) and scaled
+    to the grid."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    if not O.ref_available():
+        return None, None
+    # The reference runs in a child process of its own: its OpenMP runtime
+    # starts there with OMP_PROC_BIND=close and the thread count, and no
+    # thread placement of this (GPU) process constrains it.
+    import subprocess
+    import tempfile
+
+    with tempfile.TemporaryDirectory() as tmp:
+        fin, fout = os.path.join(tmp, "in.npy"), os.path.join(tmp, "out.npy")
+        np.save(fin, psd)
+        env = dict(os.environ, OMP_PROC_BIND="close", OMP_NUM_THREADS=str(threads), OPENBLAS_NUM_THREADS="1")
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-child", fin, fout, str(threads)],
+                           env=env, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stderr)
+            return None, None
+        t = json.loads(r.stdout.strip().splitlines()[-1])
+        x = np.load(fout)
+    est = t["conv"] + t["radial"] + t["local"]
+    got = gpu_step(psd)
+    normwise, point = node_errors(got, x)
+    check = {"max_node_err": normwise, "max_pointwise_rel_err": point, "pointwise_floor": 1e-12,
+             "against": "reference CPU step (OpenBLAS dgbtrf/dgbtrs) from the same state",
+             "bar": 1e-8}
+    cached = None
+    if t.get("cached_local") is not None:
+        tl = t["cached_local"]
+        cached = {"value": 1.0 / (t["conv"] + t["radial"] + tl), "unit": "steps/s",
+                  "seconds_per_step": t["conv"] + t["radial"] + tl,
+                  "sample": "reference LU_CACHING build: convection and radial as above, cached-factor solves of "
+                            "%d of %d nodes in %.2f s scaled to the grid" % (t["cached_nodes"], w.nP * w.nR,
+                                                                              t["cached_sample_s"])}
+    phys, logical, ncpu, model = HOST_CPU
+    base = {"value": 1.0 / est, "unit": "steps/s", "cores": threads, "kind": "reference",
+            "sample": "reference CPU path (OpenMP schedule(dynamic,1), OMP_PROC_BIND=close, OpenBLAS dgbtrf/dgbtrs "
+                      "single-threaded per call), LU caching off: one full step, convection %.2f s (all %d planes), "
+                      "radial %.2f s (all %d lines), local %.2f s (all %d nodes)"
+                      % (t["conv"], w.nV * w.nK, t["radial"], w.nP * w.nV * w.nK, t["local"], w.nP * w.nR),
+            "seconds_per_step": est, "host_cpu": model, "physical_cores": phys, "logical_cpus": logical,
+            "nproc": ncpu, "lu_caching_on": cached}
+    return base, check
+
+
+def cpu_child(fin, fout, threads, cached_nodes=64):
+    """The reference step of cpu_baseline, in its own process (no GPU):
+    one full step of the production workload from the state in fin, each
+    operator timed; the result goes to fout, the times to stdout (JSON)."""
+    import ctypes
+
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from verb4d import workload as W
+
+    w = W.config1(seed=1001)
+    psd = np.load(fin)
+    x = psd.copy()
+    t = {}
+    for op, args in (("conv", (-1, 0, 0)), ("radial", (0, -1, 0)), ("local", (0, 0, -1))):
+        t0 = time.perf_counter()
+        O.ref_step_sample(w, x, threads, *args)
+        t[op] = time.perf_counter() - t0
+    np.save(fout, x)
+    t["cached_local"] = None
+    if O.ref_available("lucache"):
+        nn = min(cached_nodes, w.nP * w.nR)
+        n = w.nV * w.nK
+        ldab = 3 * (w.nV + 1) + 1
+        lu = np.zeros(ldab * n * nn)
+        piv = np.zeros(n * nn, dtype=np.int64)
+        h = O.ref_lib("lucache")
+        g = O.grid_struct(w)
+        y = psd.copy()
+        h.ref_step_local_cached(ctypes.byref(g), O._p(y), threads, O._p(lu), piv.ctypes.data_as(O._i64p), 1, 0, nn)
+        t0 = time.perf_counter()
+        h.ref_step_local_cached(ctypes.byref(g), O._p(y), threads, O._p(lu), piv.ctypes.data_as(O._i64p), 0, 0, nn)
+        ts = time.perf_counter() - t0
+        t["cached_local"] = ts * (w.nP * w.nR) / nn
+        t["cached_nodes"] = nn
+        t["cached_sample_s"] = ts
+    print(json.dumps(t), flush=True)
+
+
+def mixed_check(w3, psd_in, got, threads, nodes=16):
+    """The reference's Diffusion_2D (oracle/_ref, FP64 OpenBLAS dgbtrf) on
+    `nodes` nodes of the stiff grid spread over (P,R), in parallel, against
+    the mixed-precision local solve's PSD from the same input."""
+    from concurrent.futures import ThreadPoolExecutor
+
     import numpy as np
 
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -95,31 +250,23 @@ def cpu_baseline(w, psd, threads, target_s):
 
     if not O.ref_available():
         return None
-    totals = {"conv": w.nV * w.nK, "radial": w.nP * w.nV * w.nK, "local": w.nP * w.nR}
-    start = {"conv": 64, "radial": 4096, "local": 16}
-    est = 0.0
-    parts = {}
-    for op in ("conv", "radial", "local"):
-        k = start[op]
-        while True:
-            k = min(k, totals[op])
-            x = psd.copy()
-            args = {"conv": (k, 0, 0), "radial": (0, k, 0), "local": (0, 0, k)}[op]
-            t0 = time.perf_counter()
-            O.ref_step_sample(w, x, threads, *args)
-            dt = time.perf_counter() - t0
-            if dt >= target_s or k == totals[op]:
-                break
-            k *= 2
-        parts[op] = (k, dt)
-        est += dt * totals[op] / k
-        del x
-    sample = ", ".join("%s %d of %d slices in %.2f s" % (op, parts[op][0], totals[op], parts[op][1])
-                       for op in parts)
-    return {"value": 1.0 / est, "unit": "steps/s", "cores": threads, "kind": "reference",
-            "sample": "reference CPU path (OpenMP schedule(dynamic,1), OpenBLAS dgbtrf/dgbtrs single-threaded "
-                      "per call): " + sample + "; scaled to one full step",
-            "seconds_per_step": est}
+    total = w3.nP * w3.nR
+    pick = [int(i) for i in np.linspace(0, total - 1, nodes)]
+
+    def one(q):
+        iP, iR = divmod(q, w3.nR)
+        o = (iP, iR)
+        ref = O.diffusion_2d(psd_in[o], w3.f["Vg"][o], w3.f["Kg"][o], w3.nV, w3.nK, w3.bc["Vl"][o], w3.bc["Vu"][o],
+                             w3.bc["Kl"][o], w3.bc["Ku"][o], (1, 1, 1, 2), w3.f["DVV"][o], w3.f["DKK"][o],
+                             w3.f["DVK"][o], w3.f["DVK"][o], w3.f["G_local"][o], w3.f["sources"][o],
+                             w3.f["losses_local"][o], w3.dt, w3.sub_dt, lib="ref")
+        return node_errors(got[iP:iP + 1, iR:iR + 1], ref[None, None])
+
+    with ThreadPoolExecutor(max(1, threads)) as ex:
+        errs = list(ex.map(one, pick))
+    return {"max_node_err": max(e[0] for e in errs), "max_pointwise_rel_err": max(e[1] for e in errs),
+            "pointwise_floor": 1e-12, "nodes_checked": len(pick),
+            "against": "reference Diffusion_2D (FP64, OpenBLAS dgbtrf/dgbtrs) on nodes spread over (P,R)"}
 
 
 def copy_calibration():
@@ -135,13 +282,20 @@ def copy_calibration():
 
 
 def main():
+    global HOST_CPU
+    HOST_CPU = host_cpu()
+    if len(sys.argv) == 5 and sys.argv[1] == "--cpu-child":
+        cpu_child(sys.argv[2], sys.argv[3], int(sys.argv[4]))
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=3.0, help="target seconds per operator sample")
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="CPU baseline threads (0: the physical cores available, within OMP_NUM_THREADS)")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the stiff 29x24x151x121 grid solved by FP32 LU + FP64 refinement")
     ap.add_argument("--no-cached", action="store_true",
                     help="skip timing steps that reuse cached LU factors (the reference's LU_CACHING mode)")
     ap.add_argument("--no-gmres", action="store_true",
@@ -279,22 +433,39 @@ def main():
         torch.cuda.empty_cache()
 
     gm = None
+    gm8 = None
     if not args.no_gmres:
         # local solve by FP32 GMRES cycles with the FP64 residual test
         # (max|b - Ax| <= 1e-11 max|b|) and band-LU fallback
-        dg.set_psd(w.psd0)
+        def time_gmres(prm):
+            dg.set_psd(w.psd0)
+            st = dg.step_gmres(prm, stats=True)
+            ctx.synchronize()
+            ev0.record(ext)
+            for _ in range(args.steps):
+                dg.step_gmres(prm)
+            ev1.record(ext)
+            ctx.synchronize()
+            torch.cuda.synchronize()
+            return st, ev0.elapsed_time(ev1) / args.steps
+
         prm = DeviceGrid.gmres_params(tol=1e-11, restart=20, max_cycles=8)
-        st = dg.step_gmres(prm, stats=True)
-        ctx.synchronize()
-        ev0.record(ext)
-        for _ in range(args.steps):
-            dg.step_gmres(prm)
-        ev1.record(ext)
-        ctx.synchronize()
-        torch.cuda.synchronize()
-        gm = {"steps_per_s": round(1000.0 * args.steps / ev0.elapsed_time(ev1), 4),
-              "ms_per_step": round(ev0.elapsed_time(ev1) / args.steps, 4), "tol": 1e-11,
-              "first_step": st}
+        st, ms = time_gmres(prm)
+        gm = {"steps_per_s": round(1000.0 / ms, 4), "ms_per_step": round(ms, 4), "tol": 1e-11, "first_step": st}
+        # the warm-start rule: at most 8 inner iterations, then the band LU
+        gm8 = {}
+        prm8 = DeviceGrid.gmres_params_capped(tol=1e-10, iterations=8, cycles=2)
+        for kp in (2.0, 6.0):
+            if kp != w.kp:
+                w.set_kp(kp)
+                dg.refresh(["DKK", "DVV", "DVK", "DLL", "VP", "VR"])
+            st, ms = time_gmres(prm8)
+            gm8["kp%g" % kp] = {"steps_per_s": round(1000.0 / ms, 4), "ms_per_step": round(ms, 4),
+                                "fallback_rate": round(st["fallback"] / max(1, st["nodes"]), 4), "first_step": st}
+        gm8["rule"] = "tol 1e-10 (FP64 residual / max|b|), 8 inner iterations in 2 FP32 cycles, then band LU"
+        if w.kp != 2.0:
+            w.set_kp(2.0)
+            dg.refresh(["DKK", "DVV", "DVK", "DLL", "VP", "VR"])
 
     uneven = None
     if not args.no_uneven:
@@ -324,11 +495,43 @@ def main():
         del dg2
         torch.cuda.empty_cache()
 
+    mixed = None
+    if not args.no_mixed:
+        # the stiff grid (kl = ku = 152): FP32 band LU + FP64 refinement
+        w3 = W.config3(seed=3001)
+        dg3 = DeviceGrid(w3, ctx=ctx)
+        mst = dg3.local_mixed(tol=1e-12, max_sweeps=8)
+        ctx.synchronize()
+        psd3_in = w3.psd0
+        mixed_out = dg3.psd_host()
+        ev0.record(ext)
+        for _ in range(args.steps):
+            dg3.step_mixed(tol=1e-12, max_sweeps=8)
+        ev1.record(ext)
+        ctx.synchronize()
+        torch.cuda.synchronize()
+        ms3 = ev0.elapsed_time(ev1) / args.steps
+        mixed = {"workload": "stiff grid 29x24x151x121, 696 band systems of 18271 unknowns (kl=ku=152), dt=600 s",
+                 "steps_per_s": round(1000.0 / ms3, 4), "ms_per_step": round(ms3, 4), "tol": 1e-12,
+                 "first_local_solve": mst}
+        del dg3
+        torch.cuda.empty_cache()
+
     base = None
+    check = None
     if not args.no_cpu_baseline:
-        threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+        threads = cpu_threads(args.cpu_threads)
         psd = dg.psd_host()
-        base = cpu_baseline(w, psd, threads, args.cpu_seconds)
+
+        def gpu_step(state):
+            dg.set_psd(state)
+            dg.step_graph()
+            dg.check_status()
+            return dg.psd_host()
+
+        base, check = cpu_baseline(w, psd, threads, gpu_step)
+        if mixed is not None:
+            mixed["check"] = mixed_check(w3, psd3_in, mixed_out, threads)
 
     value = 1000.0 / ms_per_step
     out = {
@@ -354,10 +557,16 @@ def main():
         "phases": phase_out,
         "eager_ms_per_step": round(eager_ms_per_step, 4),
     }
+    if check is not None:
+        out["validation"] = check
     if cached is not None:
         out["lu_cached"] = cached
     if gm is not None:
         out["gmres_local"] = gm
+    if gm8 is not None:
+        out["gmres_8_iterations"] = gm8
+    if mixed is not None:
+        out["mixed_precision_stiff"] = mixed
     if uneven is not None:
         out["uneven_extents"] = uneven
     if base:

@@ -44,6 +44,8 @@ class DeviceGrid:
         """Re-upload fields of the workload that changed on the host (e.g.
         after ``Workload.set_kp``)."""
         torch = self.torch
+        # the library's stream may still be reading these fields
+        self.ctx.synchronize()
         for n in names:
             src = self.w.f[n] if n in self.w.f else self.w.bc[n]
             _, tens = self.t[n]
@@ -51,6 +53,9 @@ class DeviceGrid:
         torch.cuda.synchronize()
 
     def set_psd(self, arr):
+        # the copy runs on torch's stream: wait for the library's stream first,
+        # or a step still in flight there could overwrite the new contents
+        self.ctx.synchronize()
         self.psd.copy_(self.torch.from_numpy(np.ascontiguousarray(arr, dtype=np.float64)))
         self.torch.cuda.synchronize()
 
