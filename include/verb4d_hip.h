@@ -37,7 +37,8 @@ typedef enum {
     VERB_ERR_COURANT = -3,    /* Courant number > 1 (QUICKEST6.cpp:48-52) */
     VERB_ERR_HIP = -4,        /* HIP runtime failure */
     VERB_ERR_NOMEM = -5,      /* device allocation failed */
-    VERB_ERR_NODEVICE = -6    /* no HIP device visible */
+    VERB_ERR_NODEVICE = -6,   /* no HIP device visible */
+    VERB_ERR_UNSTABLE = -7    /* DVV*DKK - DVK^2 < 0 somewhere (VERB4D_Solver.cpp:747-753) */
 } verb_status_t;
 
 /* Values of the reference's BoundaryConditionType enum class. */
@@ -63,7 +64,8 @@ size_t verb_ctx_scratch_bytes(verb_ctx* ctx);
  * No reference counterpart: the reference only times the whole run
  * (src/VERB4D_Solver.cpp:429-431, 1311-1312). */
 enum { VERB_PHASE_CONVECTION = 0, VERB_PHASE_RADIAL = 1, VERB_PHASE_LOCAL_ASSEMBLE = 2, VERB_PHASE_LOCAL_FACTOR = 3,
-       VERB_PHASE_LOCAL_SOLVE = 4, VERB_PHASE_LOCAL_KRYLOV = 5, VERB_NUM_PHASES = 6 };
+       VERB_PHASE_LOCAL_SOLVE = 4, VERB_PHASE_LOCAL_KRYLOV = 5, VERB_PHASE_LOCAL_WARMSTART = 6,
+       VERB_NUM_PHASES = 7 };
 int verb_ctx_set_timing(verb_ctx* ctx, int enable);
 int verb_ctx_get_timings(verb_ctx* ctx, double* ms, long* counts);
 
@@ -259,10 +261,35 @@ typedef struct {
     long long* negative_points;
     double* psd_lost;     /* convection's SAVE_PSD_LOST_CONV accumulator (NULL: off) */
     int fast_convection;  /* FAST_CONVECTION outer sub-steps */
+    double* vu_from_convection; /* as vl_from_convection for the V-upper boundary:
+                                   PSD[:, :, nV-1, :] when Vu_type is ConstantValue
+                                   (src/VERB4D_Solver.cpp:970-975) */
 } verb_step_options;
 int verb_step_ex(verb_ctx* ctx, const verb_grid* g, double* psd, int run_convection, int run_radial,
                  int run_local, double* lu_cache, int64_t* ipiv_cache, int recompute_lu,
                  const verb_step_options* opt);
+
+/* The scaling and limiting of UpdatableMatrix::update
+ * (src/UpdatableMatrix.cpp:601-827), on device arrays of n points: dst =
+ * original * scale (the multiplication only when scale != 1), then every
+ * point whose coordinate grid[a][i] lies outside [from[a], to[a]] on any
+ * given axis a (grid[a] != NULL) is set to 0 (MatrixLimit, :196-300). The
+ * host keeps the time queues that decide scale and ranges (and which
+ * original array a .lst update loaded). dst may equal original only when
+ * scale == 1 and no axis is given. */
+typedef struct {
+    double scale;
+    const double* grid[4];
+    double from[4], to[4];
+} verb_field_update_params;
+int verb_field_update(verb_ctx* ctx, double* dst, const double* original, size_t n,
+                      const verb_field_update_params* params);
+
+/* The stability check of the time loop (src/VERB4D_Solver.cpp:747-753):
+ * min over the grid of DVV*DKK - DVK*DVK (NaN entries passed over, as
+ * Matrix4D::min does, src/Matrix.cpp:5881-5888) into *min_value (host);
+ * returns VERB_ERR_UNSTABLE when it is negative. Synchronises the stream. */
+int verb_check_stability(verb_ctx* ctx, const verb_grid* g, double* min_value);
 
 /* Pulls the device-side status word of the last batched launches (Courant
  * violation, zero pivot) and synchronises the stream. */
@@ -295,6 +322,8 @@ typedef struct {
     int fallback;         /* solved by the band LU instead */
     long long iterations; /* inner iterations, summed over the solved nodes */
     int max_iterations;   /* inner iterations of the slowest node */
+    double max_residual;  /* largest max|b - A x| / max|b| at a node's last FP64 check */
+    double mean_log10_residual; /* mean over nodes of log10 of that residual (before any fallback) */
 } verb_gmres_stats;
 
 int verb_local_diffusion_gmres(verb_ctx* ctx, const verb_grid* g, double* psd, const verb_gmres_params* params,
@@ -362,6 +391,65 @@ int verb_local_diffusion_mixed(verb_ctx* ctx, const verb_grid* g, double* psd, c
 /* convection + radial + verb_local_diffusion_mixed */
 int verb_step_mixed(verb_ctx* ctx, const verb_grid* g, double* psd, const verb_mixed_params* params,
                     verb_mixed_stats* stats);
+
+/* ------------------------------------------------------------------------
+ * The learned warm start. No counterpart in the reference. A 4-level U-Net
+ * (3x3 convolutions 16->32->32 | 2x2 average pool | 64 | pool | 96 | pool |
+ * 128 | nearest 2x upsampling with skip concatenation: 96, 64, 32 | 1x1 head
+ * to one channel; ReLU after every 3x3 convolution; 512,065 parameters, of
+ * which 508,897 are live: the input layer's weights for the zero channels
+ * 5..15 stay 0) maps five channels of a (P,R) node on its E x alpha grid
+ * padded to multiples of 8 (Hp = round8(nV), Wp = round8(nK)):
+ *   0: (log10 Daa + 5)/3   1: (log10 DEE + 5)/3     2: DaE/sqrt(Daa DEE)
+ *   3: log10(PSD/s)/15 + 1  4: dt/600 s
+ * with Daa = DKK, DEE = DVV/E^2, DaE/sqrt(Daa DEE) = DVK/sqrt(DVV DKK) and s
+ * the node's max|PSD|, to delta = (PSD_out - PSD_in)/s; x0 = PSD_in + s delta
+ * starts the GMRES local solve. BF16 operands on MFMA with FP32
+ * accumulation; BF16 activations, FP32 gradients and master parameters.
+ * Features: feat [count][Hp][Wp][16] BF16 (uint16 bit patterns), pin
+ * [count][Hp][Wp] FP32 = PSD/s (0 in the padding), scale [count] FP64.
+ * Parameters (FP32 vector of verb_unet_param_count()): per 3x3 convolution
+ * W [9][cin][cout] (tap = 3(dh+1) + dw+1) then b [cout]; then the head
+ * w [32], b [1]. Loss of a batch: sum over pixels of
+ * (pin + delta - pout)^2 / (batch Hp Wp), pout = PSD_out/s.
+ * All device pointers; work on the ctx's stream.
+ * ------------------------------------------------------------------------ */
+typedef struct verb_unet verb_unet;
+size_t verb_unet_param_count(void);
+/* A network for batches of up to max_batch nodes of height x width
+ * (multiples of 8); train != 0 also allocates gradients and Adam state. */
+int verb_unet_create(verb_ctx* ctx, int max_batch, int height, int width, int train, verb_unet** out);
+int verb_unet_destroy(verb_unet* net);
+/* host FP32 parameter vectors */
+int verb_unet_set_params(verb_unet* net, const float* params);
+int verb_unet_get_params(verb_unet* net, float* params);
+/* features of nodes [node0, node0+count) of a device grid and PSD */
+int verb_unet_features(verb_unet* net, const verb_grid* g, const double* psd, int node0, int count, uint16_t* feat,
+                       float* pin, double* scale);
+/* out = psd / scale on the padded grid (training targets pout) */
+int verb_unet_normalize(verb_unet* net, const verb_grid* g, const double* psd, int node0, int count,
+                        const double* scale, float* out);
+/* delta [batch][height][width] */
+int verb_unet_forward(verb_unet* net, const uint16_t* feat, int batch, int height, int width, float* delta);
+/* the loss (host, nullable: then no synchronisation) and the gradient of
+ * every parameter (device FP32 vector, nullable) */
+int verb_unet_gradients(verb_unet* net, const uint16_t* feat, const float* pin, const float* pout, int batch,
+                        int height, int width, float* grad, float* loss);
+/* one Adam step (beta 0.9 / 0.999, eps 1e-8) at learning rate lr */
+int verb_unet_train_step(verb_unet* net, const uint16_t* feat, const float* pin, const float* pout, int batch,
+                         int height, int width, float lr, float* loss);
+
+/* Local diffusion from the network's warm start: x0 = PSD + s delta per node
+ * (net created on the same ctx, max_batch nodes per forward launch), then
+ * verb_local_diffusion_gmres_x0: the FP64 residual judges each node and the
+ * band LU solves the nodes that miss params->tol within the iteration
+ * budget (params->max_iterations = 8 in the north-star rule). Attaches where
+ * src/Diffusion_2D.cpp:144-148 calls Lapack. */
+int verb_local_diffusion_learned(verb_ctx* ctx, const verb_grid* g, double* psd, verb_unet* net,
+                                 const verb_gmres_params* params, verb_gmres_stats* stats);
+/* convection + radial + verb_local_diffusion_learned */
+int verb_step_learned(verb_ctx* ctx, const verb_grid* g, double* psd, verb_unet* net,
+                      const verb_gmres_params* params, verb_gmres_stats* stats);
 
 #ifdef __cplusplus
 }

@@ -24,9 +24,10 @@ SOURCES = [
     ("local.hip", ["-ffp-contract=off"]),
     ("band_lu.hip", []),
     ("gmres.hip", []),
-    ("driver.hip", []),
+    ("driver.hip", ["-ffp-contract=off"]),
     ("api.hip", []),
     ("noalloc.hip", []),
+    ("unet.hip", []),
 ]
 
 COMMON = [

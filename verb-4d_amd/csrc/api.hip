@@ -2,6 +2,7 @@
 // (host pointers, synchronous) and batched full-grid entries (device
 // pointers, asynchronous on the context's stream).
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -10,6 +11,14 @@
 
 #include "common.h"
 #include "gmres.h"
+
+namespace verb {
+// the learned warm start (unet.hip)
+int unet_warm_start(verb_unet* u, const verb_grid& g, const double* psd, int node0, int count, uint16_t* feat,
+                    float* pin, double* scale, float* delta, double* x0, hipStream_t st);
+int unet_max_batch(const verb_unet* u);
+verb_ctx* unet_ctx(const verb_unet* u);
+}  // namespace verb
 
 
 struct verb_ctx {
@@ -425,7 +434,8 @@ int gmres_local(verb_ctx* c, const verb_grid* g, double* psd, const double* x0, 
     gw.z = (float*)scratch(c, kSlotGmres + 4, 2 * n * nodes * sizeof(float));
     gw.conv = (int*)scratch(c, kSlotGmres + 5, nodes * sizeof(int));
     gw.iters = (int*)scratch(c, kSlotGmres + 6, nodes * sizeof(int));
-    if (!gw.x || !gw.d32 || !gw.pf || !gw.V || !gw.z || !gw.conv || !gw.iters)
+    gw.resid = (double*)scratch(c, kSlotGmres + 7, nodes * sizeof(double));
+    if (!gw.x || !gw.d32 || !gw.pf || !gw.V || !gw.z || !gw.conv || !gw.iters || !gw.resid)
         return set_err(c, VERB_ERR_NOMEM, "device allocation failed");
     hipEvent_t ev[6];
     hipEvent_t* marks = nullptr;
@@ -443,10 +453,13 @@ int gmres_local(verb_ctx* c, const verb_grid* g, double* psd, const double* x0, 
     if (rc) return set_err(c, rc, "GMRES local diffusion launch rejected");
     if (stats) {
         std::vector<int> conv(nodes), it(nodes);
+        std::vector<double> rs(nodes);
         HIPCHK(c, hipMemcpyAsync(conv.data(), gw.conv, nodes * sizeof(int), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipMemcpyAsync(it.data(), gw.iters, nodes * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(rs.data(), gw.resid, nodes * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
-        verb_gmres_stats st{0, 0, 0, 0, 0};
+        verb_gmres_stats st{0, 0, 0, 0, 0, 0.0, 0.0};
+        double slog = 0.0;
         for (int i = 0; i < nodes; i++) {
             const int iR = i % g->nR;
             if (!g->include_boundary && (iR == 0 || iR == g->nR - 1)) continue;
@@ -455,7 +468,10 @@ int gmres_local(verb_ctx* c, const verb_grid* g, double* psd, const double* x0, 
             else st.fallback++;
             st.iterations += it[i];
             if (it[i] > st.max_iterations) st.max_iterations = it[i];
+            if (rs[i] > st.max_residual) st.max_residual = rs[i];
+            slog += std::log10(rs[i] > 1e-300 ? rs[i] : 1e-300);
         }
+        st.mean_log10_residual = st.nodes ? slog / st.nodes : 0.0;
         *stats = st;
     }
     return VERB_OK;
@@ -474,6 +490,49 @@ int verb_local_diffusion_gmres_x0(verb_ctx* c, const verb_grid* g, double* psd, 
                                   const verb_gmres_params* prm, verb_gmres_stats* stats) {
     if (!x0) return VERB_ERR_ARG;
     return gmres_local(c, g, psd, x0, prm, stats);
+}
+
+constexpr int kSlotLearned = 100;  // scratch slots 100..104 for the warm-start workspace
+
+int verb_local_diffusion_learned(verb_ctx* c, const verb_grid* g, double* psd, verb_unet* net,
+                                 const verb_gmres_params* prm, verb_gmres_stats* stats) {
+    if (!c || !grid_ok(g) || !psd || !net || !prm) return VERB_ERR_ARG;
+    if (verb::unet_ctx(net) != c) return set_err(c, VERB_ERR_ARG, "the network belongs to another context");
+    if (!g->DKK || !g->DVV || !g->DVK || !g->Vg) return VERB_ERR_ARG;
+    const int nodes = g->nP * g->nR;
+    const int Hp = (g->nV + 7) / 8 * 8, Wp = (g->nK + 7) / 8 * 8;
+    const int chunk = std::min(nodes, verb::unet_max_batch(net));
+    const size_t n = (size_t)g->nV * g->nK;
+    uint16_t* feat;
+    float *pin, *delta;
+    double *scale, *x0;
+    {
+        std::lock_guard<std::mutex> lk(c->mu);
+        feat = (uint16_t*)scratch(c, kSlotLearned + 0, (size_t)chunk * Hp * Wp * 16 * sizeof(uint16_t));
+        pin = (float*)scratch(c, kSlotLearned + 1, (size_t)chunk * Hp * Wp * sizeof(float));
+        delta = (float*)scratch(c, kSlotLearned + 2, (size_t)chunk * Hp * Wp * sizeof(float));
+        scale = (double*)scratch(c, kSlotLearned + 3, (size_t)chunk * sizeof(double));
+        x0 = (double*)scratch(c, kSlotLearned + 4, n * nodes * sizeof(double));
+        if (!feat || !pin || !delta || !scale || !x0) return set_err(c, VERB_ERR_NOMEM, "device allocation failed");
+        const int m = mark_begin(c, VERB_PHASE_LOCAL_WARMSTART);
+        for (int node0 = 0; node0 < nodes; node0 += chunk) {
+            const int cnt = std::min(chunk, nodes - node0);
+            int rc = verb::unet_warm_start(net, *g, psd, node0, cnt, feat, pin, scale, delta, x0, c->stream);
+            if (rc) return set_err(c, rc, "warm-start network launch rejected");
+        }
+        mark_end(c, m);
+    }
+    return gmres_local(c, g, psd, x0, prm, stats);
+}
+
+int verb_step_learned(verb_ctx* c, const verb_grid* g, double* psd, verb_unet* net, const verb_gmres_params* prm,
+                      verb_gmres_stats* stats) {
+    if (!c || !grid_ok(g) || !psd) return VERB_ERR_ARG;
+    int rc = VERB_OK;
+    if (3 < g->nP || 3 < g->nR) rc = verb_convection_batched(c, g, psd);
+    if (!rc && g->nR >= 3) rc = verb_radial_diffusion_batched(c, g, psd);
+    if (!rc && g->nV >= 3 && g->nK >= 3) rc = verb_local_diffusion_learned(c, g, psd, net, prm, stats);
+    return rc;
 }
 
 constexpr int kSlotMixed = 80;  // scratch slots 80..90 for the mixed-precision workspace
@@ -752,6 +811,35 @@ int verb_step(verb_ctx* c, const verb_grid* g, double* psd, int run_convection, 
 }
 
 constexpr int kSlotDriver = 95;  // negative-point counter
+constexpr int kSlotStability = 96;  // per-block minima and the result
+
+int verb_field_update(verb_ctx* c, double* dst, const double* original, size_t n,
+                      const verb_field_update_params* prm) {
+    if (!c || !dst || !original || !prm) return VERB_ERR_ARG;
+    bool limited = false;
+    for (int a = 0; a < 4; a++) limited = limited || prm->grid[a];
+    if (dst == original && (prm->scale != 1.0 || limited))
+        return set_err(c, VERB_ERR_ARG, "field update: dst must differ from the original array");
+    if (n == 0) return VERB_OK;
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = verb::launch_field_update(dst, original, n, *prm, c->stream);
+    return rc ? set_err(c, rc, "field update launch failed") : VERB_OK;
+}
+
+int verb_check_stability(verb_ctx* c, const verb_grid* g, double* min_value) {
+    if (!c || !grid_ok(g) || !g->DVV || !g->DKK || !g->DVK) return VERB_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    double* part = (double*)scratch(c, kSlotStability, (verb::kStabilityParts + 1) * sizeof(double));
+    if (!part) return set_err(c, VERB_ERR_NOMEM, "device allocation failed");
+    int rc = verb::launch_stability_min(*g, part, part + verb::kStabilityParts, c->stream);
+    if (rc) return set_err(c, rc, "stability check launch failed");
+    double m = 0.0;
+    HIPCHK(c, hipMemcpyAsync(&m, part + verb::kStabilityParts, sizeof(m), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (min_value) *min_value = m;
+    if (m < 0) return set_err(c, VERB_ERR_UNSTABLE, "(DVV*DKK - DVK*DVK) < 0: the computation would be unstable");
+    return VERB_OK;
+}
 
 int verb_step_ex(verb_ctx* c, const verb_grid* g, double* psd, int run_convection, int run_radial, int run_local,
                  double* lu_cache, int64_t* ipiv_cache, int recompute_lu, const verb_step_options* opt) {
@@ -768,6 +856,10 @@ int verb_step_ex(verb_ctx* c, const verb_grid* g, double* psd, int run_convectio
         if (!rc && opt->vl_from_convection && g->Vl_type == VERB_BC_CONSTANT_VALUE) {
             rc = verb::launch_vl_from_psd(*g, psd, opt->vl_from_convection, c->stream);
             gl.Vl = opt->vl_from_convection;
+        }
+        if (!rc && opt->vu_from_convection && g->Vu_type == VERB_BC_CONSTANT_VALUE) {
+            rc = verb::launch_v_slice(*g, psd, g->nV - 1, opt->vu_from_convection, c->stream);
+            gl.Vu = opt->vu_from_convection;
         }
     }
     if (!rc && opt->positive_psd) rc = verb::launch_clamp_floor(psd, n, kFloor, c->stream);

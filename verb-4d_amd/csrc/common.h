@@ -102,6 +102,7 @@ struct GmresWork {
     float* z;      // [nodes][2][nfull]
     int* conv;     // [nodes]
     int* iters;    // [nodes]
+    double* resid = nullptr;  // [nodes] relative FP64 residual at the last check
 };
 // x0 (nullable, [P][R][V][K] like psd): initial iterate; null starts from psd.
 // max_iters > 0 caps the inner iterations of a node over all its cycles.
@@ -144,5 +145,10 @@ int launch_tridag(const double* a, const double* b, const double* c, const doubl
 int launch_clamp_floor(double* psd, size_t n, double floor, hipStream_t stream);
 int launch_count_negative(const double* psd, size_t n, unsigned long long* count, hipStream_t stream);
 int launch_vl_from_psd(const verb_grid& g, const double* psd, double* vl, hipStream_t stream);
+int launch_v_slice(const verb_grid& g, const double* psd, int iv, double* v, hipStream_t stream);
+int launch_field_update(double* dst, const double* src, size_t n, const verb_field_update_params& p,
+                        hipStream_t stream);
+constexpr int kStabilityParts = 1024;
+int launch_stability_min(const verb_grid& g, double* part, double* out, hipStream_t stream);
 
 }  // namespace verb

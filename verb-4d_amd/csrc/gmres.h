@@ -15,6 +15,7 @@ struct GmArgs {
     float* z;            // [nodes][2][nfull] preconditioned vector and combination scratch
     int* conv;           // [nodes] 1 when the FP64 residual met tol (or the node is skipped)
     int* iters;          // [nodes] inner iterations used
+    double* resid;       // [nodes] max|b - A x| / max|b| at the last FP64 check (nullable)
     int nx, ny, nfull;
     int m, max_cycles;
     int max_iters;       // inner iterations per node at most, over all cycles (0: m * max_cycles)

@@ -101,6 +101,7 @@ __global__ __launch_bounds__(kGmThreads) void gmres_nodes(GmArgs a) {
         if (tid == 0) {
             a.conv[node] = 1;
             a.iters[node] = 0;
+            if (a.resid) a.resid[node] = 0.0;
         }
         return;
     }
@@ -153,6 +154,7 @@ __global__ __launch_bounds__(kGmThreads) void gmres_nodes(GmArgs a) {
 
     int total = 0;
     bool ok = false;
+    double last = 0.0;  // relative FP64 residual at the last check
     for (int cyc = 0; cyc <= a.max_cycles; cyc++) {
         // r = b - A x in FP64; the FP32 copy starts the Krylov basis
         double rm = 0.0;
@@ -170,6 +172,7 @@ __global__ __launch_bounds__(kGmThreads) void gmres_nodes(GmArgs a) {
             rs[0] += r32 * r32;
         }
         const double rmax = block_max(rm, dred);
+        last = (bmax > 0.0) ? rmax / bmax : rmax;
         if (rmax <= thresh) {
             ok = true;
             break;
@@ -281,6 +284,7 @@ __global__ __launch_bounds__(kGmThreads) void gmres_nodes(GmArgs a) {
     if (tid == 0) {
         a.conv[node] = ok ? 1 : 0;
         a.iters[node] = total;
+        if (a.resid) a.resid[node] = last;
     }
 }
 

@@ -722,7 +722,7 @@ int launch_local_gmres(const verb_grid& g, double* psd, const LocalWork& w, cons
     }
     GmArgs ga;
     ga.diag = w.diag; ga.rhs = w.rhs; ga.x = gw.x; ga.d32 = gw.d32; ga.pf = gw.pf; ga.V = gw.V; ga.z = gw.z;
-    ga.conv = gw.conv; ga.iters = gw.iters;
+    ga.conv = gw.conv; ga.iters = gw.iters; ga.resid = gw.resid;
     ga.nx = nx; ga.ny = ny; ga.nfull = n; ga.m = restart; ga.max_cycles = max_cycles; ga.max_iters = max_iters;
     ga.tol = tol; ga.inner_tol = (float)inner_tol; ga.nR = g.nR; ga.include_boundary = g.include_boundary;
     int rc = launch_gmres(ga, nodes, stream);

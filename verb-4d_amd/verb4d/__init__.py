@@ -50,7 +50,13 @@ class StepOptions(ctypes.Structure):
     convection and negative-point count around the step."""
     _fields_ = [("positive_psd", ctypes.c_int), ("vl_from_convection", ctypes.c_void_p),
                 ("negative_points", ctypes.POINTER(ctypes.c_longlong)), ("psd_lost", ctypes.c_void_p),
-                ("fast_convection", ctypes.c_int)]
+                ("fast_convection", ctypes.c_int), ("vu_from_convection", ctypes.c_void_p)]
+
+
+class FieldUpdateParams(ctypes.Structure):
+    """verb_field_update_params: scaling and per-axis limits of an update."""
+    _fields_ = [("scale", ctypes.c_double), ("grid", ctypes.c_void_p * 4), ("from_", ctypes.c_double * 4),
+                ("to", ctypes.c_double * 4)]
 
 
 class ConvectionOptions(ctypes.Structure):
@@ -70,7 +76,8 @@ class GmresStats(ctypes.Structure):
     """ctypes mirror of ``verb_gmres_stats``."""
 
     _fields_ = (("nodes", ctypes.c_int), ("converged", ctypes.c_int), ("fallback", ctypes.c_int),
-                ("iterations", ctypes.c_longlong), ("max_iterations", ctypes.c_int))
+                ("iterations", ctypes.c_longlong), ("max_iterations", ctypes.c_int),
+                ("max_residual", ctypes.c_double), ("mean_log10_residual", ctypes.c_double))
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -185,6 +192,8 @@ _SIGNATURES = [
      [_P, ctypes.POINTER(VerbGrid), ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
       ctypes.c_void_p, ctypes.c_int]),
     ("verb_check_status", ctypes.c_int, [_P]),
+    ("verb_field_update", ctypes.c_int, [_P, _P, _P, ctypes.c_size_t, ctypes.POINTER(FieldUpdateParams)]),
+    ("verb_check_stability", ctypes.c_int, [_P, ctypes.POINTER(VerbGrid), ctypes.POINTER(ctypes.c_double)]),
     ("verb_local_diffusion_gmres", ctypes.c_int,
      [_P, ctypes.POINTER(VerbGrid), ctypes.c_void_p, ctypes.POINTER(GmresParams), ctypes.POINTER(GmresStats)]),
     ("verb_local_diffusion_gmres_x0", ctypes.c_int,
@@ -200,6 +209,24 @@ _SIGNATURES = [
     ("verb_get_tuning", ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),
     ("verb_autotune", ctypes.c_int, [_P, ctypes.POINTER(VerbGrid), ctypes.c_void_p, ctypes.POINTER(Tuning)]),
     ("verb_step_graph", ctypes.c_int, [_P, ctypes.POINTER(VerbGrid), ctypes.c_void_p, ctypes.c_void_p]),
+    ("verb_unet_param_count", ctypes.c_size_t, []),
+    ("verb_unet_create", ctypes.c_int,
+     [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    ("verb_unet_destroy", ctypes.c_int, [_P]),
+    ("verb_unet_set_params", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_float)]),
+    ("verb_unet_get_params", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_float)]),
+    ("verb_unet_features", ctypes.c_int,
+     [_P, ctypes.POINTER(VerbGrid), _P, ctypes.c_int, ctypes.c_int, _P, _P, _P]),
+    ("verb_unet_normalize", ctypes.c_int, [_P, ctypes.POINTER(VerbGrid), _P, ctypes.c_int, ctypes.c_int, _P, _P]),
+    ("verb_unet_forward", ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P]),
+    ("verb_unet_gradients", ctypes.c_int,
+     [_P, _P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, ctypes.POINTER(ctypes.c_float)]),
+    ("verb_unet_train_step", ctypes.c_int,
+     [_P, _P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.POINTER(ctypes.c_float)]),
+    ("verb_local_diffusion_learned", ctypes.c_int,
+     [_P, ctypes.POINTER(VerbGrid), _P, _P, ctypes.POINTER(GmresParams), ctypes.POINTER(GmresStats)]),
+    ("verb_step_learned", ctypes.c_int,
+     [_P, ctypes.POINTER(VerbGrid), _P, _P, ctypes.POINTER(GmresParams), ctypes.POINTER(GmresStats)]),
     ("verb_step_ex", ctypes.c_int,
      [_P, ctypes.POINTER(VerbGrid), ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
       ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(StepOptions)]),
@@ -254,7 +281,8 @@ class Context:
     def check_status(self):
         return self.check(self._lib.verb_check_status(self.handle))
 
-    PHASES = ("convection", "radial", "local_assemble", "local_factor", "local_solve", "local_krylov")
+    PHASES = ("convection", "radial", "local_assemble", "local_factor", "local_solve", "local_krylov",
+              "local_warmstart")
 
     def set_timing(self, on=True):
         self.check(self._lib.verb_ctx_set_timing(self.handle, 1 if on else 0))
