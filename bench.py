@@ -269,6 +269,94 @@ def mixed_check(w3, psd_in, got, threads, nodes=16):
             "against": "reference Diffusion_2D (FP64, OpenBLAS dgbtrf/dgbtrs) on nodes spread over (P,R)"}
 
 
+def learned_sequence(ctx, ext, ev0, ev1, nsteps, tsteps):
+    """The learned warm start on the held-out sequence: the production grid
+    (seed 4001, not in the training data) with Kp 1, 8 and 2 for a third of
+    the steps each. Two copies of the PSD advance side by side: one by the
+    learned step (network x0, FP32 GMRES judged by the FP64 residual at 1e-10,
+    band LU for every node not converged within 8 inner iterations), one by
+    the plain FP64 step (the parity-checked band LU path). Reports the
+    fallback rate per Kp segment, the largest per-node normwise and
+    pointwise (|ref| >= 1e-12 node max) difference to the FP64 path over the
+    sequence, and steps/s of both paths timed on tsteps steps at each Kp."""
+    import torch
+
+    from verb4d import learned as L
+    from verb4d import workload as W
+    from verb4d.device import DeviceGrid
+
+    if not os.path.exists(L.WEIGHTS):
+        return None
+    params, meta = L.load()
+    w = W.config1(seed=4001)
+    w.set_kp(1.0)
+    dl = DeviceGrid(w, ctx=ctx)
+    dp = DeviceGrid(w, ctx=ctx)
+    net = L.UNet(ctx, 696, 104, 96, train=False, params=params)
+    prm = DeviceGrid.gmres_params_capped()
+    fields = ["DKK", "DVV", "DVK", "DLL", "VP", "VR"]
+    seg = {}
+    worst_n, worst_p = 0.0, 0.0
+    third = max(1, nsteps // 3)
+    kps = [1.0, 8.0, 2.0]
+    for it in range(nsteps):
+        kp = kps[min(2, it // third)]
+        if kp != w.kp:
+            w.set_kp(kp)
+            dl.refresh(fields)
+            dp.refresh(fields)
+        st = L.step_learned(dl, net, prm, stats=True)
+        dp.step()
+        s = seg.setdefault("kp%g" % kp, {"steps": 0, "nodes": 0, "fallback": 0, "iterations": 0})
+        s["steps"] += 1
+        s["nodes"] += st["nodes"]
+        s["fallback"] += st["fallback"]
+        s["iterations"] += st["iterations"]
+        if it % 16 == 15 or it == nsteps - 1:
+            a, b = dl.psd_host(), dp.psd_host()
+            en, ep = node_errors(a, b)
+            worst_n, worst_p = max(worst_n, en), max(worst_p, ep)
+    dl.check_status()
+    dp.check_status()
+    for s in seg.values():
+        s["fallback_rate"] = round(s["fallback"] / max(1, s["nodes"]), 4)
+        s["mean_iterations"] = round(s["iterations"] / max(1, s["nodes"]), 2)
+    # timing at each Kp, without per-step statistics
+    timing = {}
+    for kp in kps:
+        w.set_kp(kp)
+        dl.refresh(fields)
+        dp.refresh(fields)
+        for name, fn in (("learned", lambda: L.step_learned(dl, net, prm)), ("plain", lambda: dp.step())):
+            fn()
+            ctx.synchronize()
+            ev0.record(ext)
+            for _ in range(tsteps):
+                fn()
+            ev1.record(ext)
+            ctx.synchronize()
+            torch.cuda.synchronize()
+            timing.setdefault(name, {})["kp%g" % kp] = round(1000.0 * tsteps / ev0.elapsed_time(ev1), 4)
+    phases = {}
+    ctx.set_timing(True)
+    ctx.timings()
+    L.step_learned(dl, net, prm)
+    for k, (ms, cnt) in ctx.timings().items():
+        if cnt:
+            phases[k] = round(ms, 3)
+    ctx.set_timing(False)
+    mean = lambda d: sum(d.values()) / len(d)  # noqa: E731
+    return {"workload": "production grid 29x24x101x91, seed 4001, Kp 1 -> 8 -> 2 over %d steps" % nsteps,
+            "rule": "x0 from the U-Net, FP32 GMRES (2 cycles of 4) judged by the FP64 residual at 1e-10, band LU "
+                    "for nodes not converged within 8 inner iterations",
+            "segments": seg, "max_node_err_vs_fp64": worst_n, "max_pointwise_rel_err_vs_fp64": worst_p,
+            "steps_per_s": timing["learned"], "plain_steps_per_s": timing["plain"],
+            "speedup_vs_plain": round(mean(timing["learned"]) / mean(timing["plain"]), 3),
+            "phase_ms_one_step": phases,
+            "network": {"parameters": int(params.size), "training_samples": int(meta.get("samples", 0)),
+                        "epochs": int(meta.get("epochs", 0))}}
+
+
 def copy_calibration():
     import torch
     n = 64 << 20
@@ -296,6 +384,10 @@ def main():
                     help="CPU baseline threads (0: the physical cores available, within OMP_NUM_THREADS)")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the stiff 29x24x151x121 grid solved by FP32 LU + FP64 refinement")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned warm start on the held-out Kp 1 -> 8 -> 2 sequence")
+    ap.add_argument("--learned-steps", type=int, default=288,
+                    help="steps of the held-out Kp sequence (thirds at Kp 1, 8, 2)")
     ap.add_argument("--no-cached", action="store_true",
                     help="skip timing steps that reuse cached LU factors (the reference's LU_CACHING mode)")
     ap.add_argument("--no-gmres", action="store_true",
@@ -495,6 +587,10 @@ def main():
         del dg2
         torch.cuda.empty_cache()
 
+    learned = None
+    if not args.no_learned:
+        learned = learned_sequence(ctx, ext, ev0, ev1, args.learned_steps, args.steps)
+
     mixed = None
     if not args.no_mixed:
         # the stiff grid (kl = ku = 152): FP32 band LU + FP64 refinement
@@ -567,6 +663,8 @@ def main():
         out["gmres_8_iterations"] = gm8
     if mixed is not None:
         out["mixed_precision_stiff"] = mixed
+    if learned is not None:
+        out["learned"] = learned
     if uneven is not None:
         out["uneven_extents"] = uneven
     if base:
