@@ -38,7 +38,11 @@ typedef enum {
     VERB_ERR_HIP = -4,        /* HIP runtime failure */
     VERB_ERR_NOMEM = -5,      /* device allocation failed */
     VERB_ERR_NODEVICE = -6,   /* no HIP device visible */
-    VERB_ERR_UNSTABLE = -7    /* DVV*DKK - DVK^2 < 0 somewhere (VERB4D_Solver.cpp:747-753) */
+    VERB_ERR_UNSTABLE = -7,   /* DVV*DKK - DVK^2 < 0 somewhere (VERB4D_Solver.cpp:747-753) */
+    VERB_ERR_DOMAIN = -8      /* interpolation input the reference rejects: a non-finite point to
+                                 evaluate or NaN setting the data's direction (the interpolator
+                                 throws, MonotCubicInterpolator.cpp:178-180, 464), or a NaN grid
+                                 value (no place in std::map's order) */
 } verb_status_t;
 
 /* Values of the reference's BoundaryConditionType enum class. */
@@ -291,6 +295,20 @@ int verb_field_update(verb_ctx* ctx, double* dst, const double* original, size_t
  * returns VERB_ERR_UNSTABLE when it is negative. Synchronises the stream. */
 int verb_check_stability(verb_ctx* ctx, const verb_grid* g, double* min_value);
 
+/* Adiabatic remapping (src/VERB4D_Solver.cpp:557-619): every (P,V,K) line of
+ * psd is interpolated from the old L* values of its points to the new ones
+ * with Cubic1D (src/Interpolation.cpp:25-51: the monotone cubic Hermite
+ * interpolator of src/MonotCubicInterpolator.cpp over the points (0, 0) and
+ * (lstar_old[i], psd[i]), the extra point coming from Cubic1D's vectors
+ * holding n zeros before the grid). psd, lstar_old and lstar_new are device
+ * arrays [P][R][V][K]; nR <= 105. Asynchronous on the context's stream; a
+ * VERB_ERR_DOMAIN line is left unchanged and reported by verb_check_status. */
+int verb_remap_batched(verb_ctx* ctx, double* psd, const double* lstar_old, const double* lstar_new, int nP,
+                       int nR, int nV, int nK);
+/* Cubic1D on one line of host arrays (src/Interpolation.h:25): values (n)
+ * replaced by the interpolant at x_new (n); synchronous. */
+int verb_cubic1d(verb_ctx* ctx, int n, const double* x_old, double* values, const double* x_new);
+
 /* Pulls the device-side status word of the last batched launches (Courant
  * violation, zero pivot) and synchronises the stream. */
 int verb_check_status(verb_ctx* ctx);
@@ -443,7 +461,7 @@ int verb_unet_train_step(verb_unet* net, const uint16_t* feat, const float* pin,
  * (net created on the same ctx, max_batch nodes per forward launch), then
  * verb_local_diffusion_gmres_x0: the FP64 residual judges each node and the
  * band LU solves the nodes that miss params->tol within the iteration
- * budget (params->max_iterations = 8 in the north-star rule). Attaches where
+ * budget (params->max_iterations = 8: the 8-iteration rule). Attaches where
  * src/Diffusion_2D.cpp:144-148 calls Lapack. */
 int verb_local_diffusion_learned(verb_ctx* ctx, const verb_grid* g, double* psd, verb_unet* net,
                                  const verb_gmres_params* params, verb_gmres_stats* stats);

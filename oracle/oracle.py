@@ -52,6 +52,8 @@ REF_VARIANTS = {
     "ref": REF_LIB,
     "hipops": os.path.join(HERE, "_ref", "libverbref_hipops.so"),
     "hiplapack": os.path.join(HERE, "_ref", "libverbref_hiplapack.so"),
+    # the reference's remapping loop calling the GPU Cubic1D drop-in
+    "hipremap": os.path.join(HERE, "_ref", "libverbref_hipremap.so"),
     # the reference built with its compile-time options on this path
     "lucache": os.path.join(HERE, "_ref", "libverbref_lucache.so"),
     "lost": os.path.join(HERE, "_ref", "libverbref_lost.so"),
@@ -79,6 +81,8 @@ def oracle_lib():
         h.orc_convection_2d_ex.argtypes = [_dp, _dp, _dp, _dp, _I, _I, _dp, _dp, _dp, _dp, _I, _I, _I, _I] + [
             _dp] * 5 + [_D, _I]
         h.orc_step.argtypes = [ctypes.POINTER(Grid), _dp, _I, _I, _I, _I]
+        h.orc_cubic1d.argtypes = [_I, _dp, _dp, _dp]
+        h.orc_remap.argtypes = [_dp, _dp, _dp, _I, _I, _I, _I]
         h.orc_grid_struct_size.restype = ctypes.c_size_t
         assert h.orc_grid_struct_size() == ctypes.sizeof(Grid)
         _oracle = h
@@ -112,12 +116,15 @@ def ref_lib(variant="ref"):
                                 ctypes.c_long, ctypes.c_long, ctypes.c_long, ctypes.c_long],
             "ref_step_local_cached": [ctypes.POINTER(Grid), _dp, _I, _dp, _i64p, _I, ctypes.c_long,
                                       ctypes.c_long],
+            "ref_cubic1d": [_I, _dp, _dp, _dp],
+            "ref_remap": [_dp, _dp, _dp, _I, _I, _I, _I],
         }
         for name, args in sigs.items():
             if hasattr(h, name):
                 getattr(h, name).argtypes = args
-        h.ref_grid_struct_size.restype = ctypes.c_size_t
-        assert h.ref_grid_struct_size() == ctypes.sizeof(Grid)
+        if hasattr(h, "ref_grid_struct_size"):
+            h.ref_grid_struct_size.restype = ctypes.c_size_t
+            assert h.ref_grid_struct_size() == ctypes.sizeof(Grid)
         _ref[variant] = h
     return _ref[variant]
 
@@ -294,3 +301,26 @@ def ref_step_sample(w, psd, nthreads, q, l, nn):
     [0,nn) (timing sample); psd is updated in place."""
     g = grid_struct(w)
     return ref_lib().ref_step_sample(ctypes.byref(g), _p(psd), nthreads, 1, 1, 1, 0, q, 0, l, 0, nn)
+
+
+def cubic1d(x_old, values, x_new, lib="oracle"):
+    """Cubic1D (src/Interpolation.cpp:25-51): values on x_old interpolated
+    to x_new (same length); returns (status, result)."""
+    xo, xn = _c(x_old), _c(x_new)
+    out = _c(values).copy()
+    assert xo.size == xn.size == out.size
+    f = oracle_lib().orc_cubic1d if lib == "oracle" else ref_lib(lib).ref_cubic1d
+    st = f(out.size, _p(xo), _p(out), _p(xn))
+    return st, out
+
+
+def remap(psd, l_old, l_new, lib="oracle"):
+    """The remapping loop (src/VERB4D_Solver.cpp:574-606) on a copy of psd
+    [P][R][V][K]; returns (status, result)."""
+    out = _c(psd).copy()
+    lo, ln = _c(l_old), _c(l_new)
+    nP, nR, nV, nK = out.shape
+    assert lo.shape == ln.shape == out.shape
+    f = oracle_lib().orc_remap if lib == "oracle" else ref_lib(lib).ref_remap
+    st = f(_p(out), _p(lo), _p(ln), nP, nR, nV, nK)
+    return st, out

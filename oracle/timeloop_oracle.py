@@ -5,7 +5,7 @@ verb4d.timeloop (the product drives the same sequence on the device).
   UpdatableArray.update   src/UpdatableMatrix.cpp:601-827 (file queue
                           :636-663, scale queue :674-713, limit queues
                           :715-800, MatrixLimit :269-300)
-  step                    src/VERB4D_Solver.cpp:621 (clamp), 633-740
+  step                    src/VERB4D_Solver.cpp:557-619 (remap), 621 (clamp), 633-740
                           (updates), 747-753 (stability check), 851-978
                           (convection, Vl/Vu from convection :965-975),
                           980 / 1072 (clamps), 990-1216 (radial, local),
@@ -71,11 +71,23 @@ def stability_min(w):
     return float(v.min()) if v.size else float("inf")
 
 
-def step(w, psd, fields, t, positive_psd=False, vl_from_convection=False, vu_from_convection=False, threads=8):
+def step(w, psd, fields, t, positive_psd=False, vl_from_convection=False, vu_from_convection=False, threads=8,
+         remap=None):
     """One step of the loop on the workload w (its arrays are updated in
-    place from `fields` {name: UpdatableArray}); returns (psd, info)."""
+    place from `fields` {name: UpdatableArray}); returns (psd, info).
+    remap: {"lstar": UpdatableArray, "copy": previous L*, "jacobians":
+    {name: UpdatableArray}} (updated in place), src/VERB4D_Solver.cpp:557-619."""
     floor = 1e-21
     x = np.array(psd, copy=True)
+    remapped = False
+    if remap is not None and remap["lstar"].update(t):
+        for n, f in remap["jacobians"].items():
+            f.update(t)
+            w.f[n][...] = f.current
+        st, x = O.remap(x, remap["copy"], remap["lstar"].current)
+        assert st == 0, st
+        remap["copy"] = remap["lstar"].current.copy()
+        remapped = True
     if positive_psd:
         x = np.where(x < floor, floor, x)
     updated = {}
@@ -102,4 +114,4 @@ def step(w, psd, fields, t, positive_psd=False, vl_from_convection=False, vu_fro
         x = np.where(x < floor, floor, x)
     else:
         neg = int((x < 0).sum())
-    return x, {"updated": updated, "negative_points": neg, "stability_min": m}
+    return x, {"updated": updated, "negative_points": neg, "stability_min": m, "remapped": remapped}

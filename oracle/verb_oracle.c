@@ -821,3 +821,141 @@ int orc_step(const orc_grid* g, double* psd, int nthreads, int run_convection, i
 }
 
 size_t orc_grid_struct_size(void) { return sizeof(orc_grid); }
+
+/* ------------------------------------------------------------------------
+ * Adiabatic remapping: Cubic1D (src/Interpolation.cpp:25-51) over
+ * Opm::MonotCubicInterpolator (src/MonotCubicInterpolator.cpp), and the
+ * loop over (P,V,K) lines of src/VERB4D_Solver.cpp:574-606.
+ *
+ * Cubic1D builds its x and f vectors with n leading zero entries (sized n,
+ * then push_back, :33-40), so the interpolator holds the point (0, 0)
+ * followed by the old grid. The constructor (:92-106) stores the points in
+ * a std::map: keys sorted and unique, a repeated key takes the value
+ * inserted last. computeInternalFunctionData (:378-497) decides
+ * monotonicity, computeSimpleDerivatives (:612-664) takes one-sided slopes
+ * at the ends and the mean of the two half-slopes inside,
+ * adjustDerivativesForMonotoneness (:668-704) applies the Fritsch-Carlson
+ * limit interval by interval (a derivative changed for interval s is the
+ * left one of interval s+1), and evaluate (:175-233) extrapolates
+ * constants and evaluates the cubic Hermite form with the bases of
+ * src/MonotCubicInterpolator.hpp:522-537.
+ * Status: 0; -8 where the reference throws (a non-finite new-grid value,
+ * NaN in the pair that fixes the direction) or where std::map's ordering is
+ * undefined (a NaN old-grid value).
+ * ---------------------------------------------------------------------- */
+#define ORC_ERR_DOMAIN (-8)
+
+static double h00(double t) { return 2 * t * t * t - 3 * t * t + 1; }
+static double h10(double t) { return t * t * t - 2 * t * t + t; }
+static double h01(double t) { return -2 * t * t * t + 3 * t * t; }
+static double h11(double t) { return t * t * t - t * t; }
+
+/* values (length n) is replaced by the interpolant at x_new. */
+int orc_cubic1d(int n, const double* x_old, double* values, const double* x_new) {
+    if (n <= 0) return 0;
+    for (int i = 0; i < n; i++)
+        if (isnan(x_old[i]) || !isfinite(x_new[i])) return ORC_ERR_DOMAIN;
+    double* k = (double*)malloc(sizeof(double) * 3 * (size_t)(n + 1));
+    double* v = k + n + 1;
+    double* d = v + n + 1;
+    int m = 1;
+    k[0] = 0.0;
+    v[0] = 0.0;
+    for (int i = 0; i < n; i++) {
+        const double x = x_old[i], f = values[i];
+        int pos = m;
+        while (pos > 0 && x < k[pos - 1]) pos--;
+        if (pos > 0 && k[pos - 1] == x) {
+            v[pos - 1] = f;
+        } else {
+            for (int s = m; s > pos; s--) {
+                k[s] = k[s - 1];
+                v[s] = v[s - 1];
+            }
+            k[pos] = x;
+            v[pos] = f;
+            m++;
+        }
+    }
+    if (m >= 2) {
+        int mono = 1, s = 0;
+        while (s + 1 < m && v[s] == v[s + 1]) s++;
+        if (s + 1 < m) {
+            if (v[s] > v[s + 1]) {
+                for (s++; s + 1 < m; s++)
+                    if (v[s] < v[s + 1]) { mono = 0; break; }
+            } else if (v[s] < v[s + 1]) {
+                for (s++; s + 1 < m; s++)
+                    if (v[s] > v[s + 1]) { mono = 0; break; }
+            } else {
+                free(k);
+                return ORC_ERR_DOMAIN; /* "Programming logic error." */
+            }
+        }
+        d[0] = (v[1] - v[0]) / (k[1] - k[0]);
+        d[m - 1] = (v[m - 1] - v[m - 2]) / (k[m - 1] - k[m - 2]);
+        for (s = 1; s < m - 1; s++)
+            d[s] = (v[s + 1] - v[s]) / (2 * (k[s + 1] - k[s])) + (v[s] - v[s - 1]) / (2 * (k[s] - k[s - 1]));
+        if (mono) {
+            for (s = 0; s < m - 1; s++) {
+                const double delta = (v[s + 1] - v[s]) / (k[s + 1] - k[s]);
+                if (fabs(delta) < 1e-14) {
+                    d[s] = 0.0;
+                    d[s + 1] = 0.0;
+                } else {
+                    const double alpha = d[s] / delta, beta = d[s + 1] / delta;
+                    if (!((alpha * alpha + beta * beta) <= 9)) {
+                        const double tau = 3 / sqrt(alpha * alpha + beta * beta);
+                        d[s] = tau * alpha * delta;
+                        d[s + 1] = tau * beta * delta;
+                    }
+                }
+            }
+        }
+    }
+    for (int i = 0; i < n; i++) {
+        const double x = x_new[i];
+        int lo = 0, hi = m;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (k[mid] < x) lo = mid + 1;
+            else hi = mid;
+        }
+        double r;
+        if (lo == 0) r = v[0];
+        else if (lo == m) r = v[m - 1];
+        else {
+            const double x1 = k[lo - 1], x2 = k[lo];
+            const double t = (x - x1) / (x2 - x1);
+            const double h = x2 - x1;
+            r = v[lo - 1] * h00(t) + d[lo - 1] * h10(t) * h + v[lo] * h01(t) + d[lo] * h11(t) * h;
+        }
+        values[i] = r;
+    }
+    free(k);
+    return 0;
+}
+
+/* The remapping loop: every (P,V,K) line of psd [P][R][V][K] is
+ * interpolated from l_old to l_new (same layout). Returns the first nonzero
+ * status. */
+int orc_remap(double* psd, const double* l_old, const double* l_new, int nP, int nR, int nV, int nK) {
+    const size_t NQ = (size_t)nV * nK;
+    int status = 0;
+    double* buf = (double*)malloc(sizeof(double) * 3 * (size_t)(nR > 0 ? nR : 1));
+    for (int iP = 0; iP < nP && !status; iP++)
+        for (size_t q = 0; q < NQ && !status; q++) {
+            double *xo = buf, *f = buf + nR, *xn = buf + 2 * nR;
+            const size_t base = (size_t)iP * nR * NQ + q;
+            for (int i = 0; i < nR; i++) {
+                xo[i] = l_old[base + i * NQ];
+                f[i] = psd[base + i * NQ];
+                xn[i] = l_new[base + i * NQ];
+            }
+            status = orc_cubic1d(nR, xo, f, xn);
+            if (!status)
+                for (int i = 0; i < nR; i++) psd[base + i * NQ] = f[i];
+        }
+    free(buf);
+    return status;
+}

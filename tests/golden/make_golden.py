@@ -3,8 +3,9 @@ reference's own hot-path code (oracle/_ref/libverbref.so, compiled by
 oracle/Makefile from the PRBEM/VERB-4D sources) on small seeded inputs.
 
 Run in a container that holds the reference tree:
-    make -C oracle && python tests/golden/make_golden.py [options]
-("options" writes only the fixtures of the reference's build options).
+    make -C oracle && python tests/golden/make_golden.py [options|remap]
+("options" writes only the fixtures of the reference's build options,
+"remap" only those of the adiabatic remapping).
 The fixtures are data only (inputs and the reference's outputs).
 """
 import os
@@ -192,9 +193,73 @@ def option_fixtures():
     save("local_lucache_node.npz", lucache_case(ws, 1, 1, 100.0))
 
 
+def remap_line_cases(seed, count):
+    """Lines of every kind the remapping meets: sorted, unsorted and repeated
+    old grids, zero and negative keys, an infinite key, monotone increasing,
+    decreasing, constant and tiny profiles (slopes under 1e-14), a NaN value,
+    new grids on the knots and beyond both ends, a non-finite new value."""
+    rng = np.random.default_rng(seed)
+    xs, fs, ns, outs, lens, sts = [], [], [], [], [], []
+    for trial in range(count):
+        n = int(rng.integers(1, 40))
+        kind = trial % 10
+        xo = np.sort(rng.uniform(1, 7, n))
+        if kind == 1:
+            xo = rng.uniform(1, 7, n)
+        if kind == 2:
+            xo = np.round(rng.uniform(1, 4, n) * 2) / 2
+        if kind == 3:
+            xo = rng.uniform(-2, 3, n)
+            xo[rng.integers(0, n)] = 0.0
+        if kind == 4:
+            xo[rng.integers(0, n)] = np.inf
+        f = rng.uniform(0, 1, n) * 10.0 ** rng.uniform(-25, 0)
+        if kind == 5:
+            f = np.sort(f)
+        if kind == 6:
+            f = np.sort(f)[::-1].copy()
+        if kind == 7:
+            f = np.full(n, 3e-7)
+        if kind == 8:
+            f = np.exp(-(xo - 4.0) ** 2) * 1e-15
+        if kind == 9:
+            f[rng.integers(0, n)] = np.nan
+        xn = xo * (1 + 0.1 * rng.standard_normal(n))
+        if trial % 7 == 0:
+            xn = xo.copy()
+        if trial % 11 == 0:
+            xn = rng.uniform(-1, 9, n)
+        if trial % 13 == 0:
+            xn[rng.integers(0, n)] = np.nan
+        st, out = O.cubic1d(xo, f, xn, lib="ref")
+        xs.append(xo)
+        fs.append(f)
+        ns.append(xn)
+        outs.append(out if st == 0 else np.full(n, np.nan))
+        lens.append(n)
+        sts.append(st)
+    return dict(x_old=np.concatenate(xs), f=np.concatenate(fs), x_new=np.concatenate(ns),
+                out=np.concatenate(outs), n=np.array(lens), status=np.array(sts))
+
+
+def remap_fixtures():
+    """Cubic1D lines and one remapped grid (oracle/_ref/libverbref.so)."""
+    if not O.ref_available():
+        O.build()
+    save("remap_lines.npz", remap_line_cases(21, 300))
+    w = W.Workload(nP=4, nR=10, nV=21, nK=17, seed=3)
+    lo, ln = W.lstar(w, 0.05), W.lstar(w, 0.12)
+    st, out = O.remap(w.psd0, lo, ln, lib="ref")
+    assert st == 0
+    save("remap_grid.npz", dict(nP=4, nR=10, nV=21, nK=17, seed=3, c_old=0.05, c_new=0.12, out=out))
+
+
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "options":
         option_fixtures()
+        return
+    if len(sys.argv) > 1 and sys.argv[1] == "remap":
+        remap_fixtures()
         return
     if not O.ref_available():
         O.build()

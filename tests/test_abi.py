@@ -70,11 +70,12 @@ def test_grid_struct_layout_matches_oracle():
 
 @pytest.mark.parametrize("tu,define", [("Convection_2D_hip.cpp", None), ("Convection_2D_hip.cpp", "SAVE_PSD_LOST_CONV"),
                                        ("Convection_2D_hip.cpp", "FAST_CONVECTION"),
-                                       ("Diffusion_2D_hip.cpp", None), ("Diffusion_2D_hip.cpp", "LU_CACHING")])
+                                       ("Diffusion_2D_hip.cpp", None), ("Diffusion_2D_hip.cpp", "LU_CACHING"),
+                                       ("Interpolation_hip.cpp", None)])
 def test_dropins_compile_under_reference_build_options(tu, define):
     """Each C++ drop-in compiles against the reference's headers with the
     build option that selects its signature (src/Convection_2D.h:84-109,
-    src/Diffusion_2D.h:86-111)."""
+    src/Diffusion_2D.h:86-111; src/Interpolation.h:25 has one)."""
     import shutil
     import subprocess
     ref = "/root/reference/src"

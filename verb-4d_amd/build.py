@@ -25,6 +25,7 @@ SOURCES = [
     ("band_lu.hip", []),
     ("gmres.hip", []),
     ("driver.hip", ["-ffp-contract=off"]),
+    ("remap.hip", ["-ffp-contract=off"]),
     ("api.hip", []),
     ("noalloc.hip", []),
     ("unet.hip", []),

@@ -99,6 +99,8 @@ int finish(verb_ctx* c) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (st[0] == VERB_ERR_COURANT) return set_err(c, VERB_ERR_COURANT, "max(CourNum) > 1, calculation can't be performed");
     if (st[0] == VERB_ERR_BC) return set_err(c, VERB_ERR_BC, "unknown boundary type");
+    if (st[0] == VERB_ERR_DOMAIN)
+        return set_err(c, VERB_ERR_DOMAIN, "MonotCubicInterpolator: non-finite input the interpolator rejects");
     if (st[0] != 0) return set_err(c, st[0], "device error");
     if (st[1] != 0x7fffffff) return set_err(c, st[1], "Error computing LU decomposition (zero pivot)");
     return VERB_OK;
@@ -838,6 +840,36 @@ int verb_check_stability(verb_ctx* c, const verb_grid* g, double* min_value) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (min_value) *min_value = m;
     if (m < 0) return set_err(c, VERB_ERR_UNSTABLE, "(DVV*DKK - DVK*DVK) < 0: the computation would be unstable");
+    return VERB_OK;
+}
+
+int verb_remap_batched(verb_ctx* c, double* psd, const double* lstar_old, const double* lstar_new, int nP, int nR,
+                       int nV, int nK) {
+    if (!c || !psd || !lstar_old || !lstar_new || nP < 0 || nR < 0 || nV < 0 || nK < 0) return VERB_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = verb::launch_remap(psd, lstar_old, lstar_new, nP, nR, nV, nK, c->status, c->stream);
+    return rc ? set_err(c, rc, "remap launch rejected") : VERB_OK;
+}
+
+int verb_cubic1d(verb_ctx* c, int n, const double* x_old, double* values, const double* x_new) {
+    if (!c || n < 0 || (n > 0 && (!x_old || !values || !x_new))) return VERB_ERR_ARG;
+    if (n == 0) return VERB_OK;
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = 0;
+    const double* dxo = up(c, 1, x_old, n, rc);
+    const double* dxn = up(c, 2, x_new, n, rc);
+    double* dv = (double*)scratch(c, 0, n * sizeof(double));
+    if (rc) return rc;
+    if (!dv) return set_err(c, VERB_ERR_NOMEM, "device allocation failed");
+    HIPCHK(c, hipMemcpyAsync(dv, values, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if ((rc = reset_status(c))) return rc;
+    rc = verb::launch_remap(dv, dxo, dxn, 1, n, 1, 1, c->status, c->stream);
+    if (rc) return set_err(c, rc, "remap launch rejected");
+    rc = finish(c);
+    reset_status(c);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(values, dv, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return VERB_OK;
 }
 

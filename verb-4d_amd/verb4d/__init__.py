@@ -194,6 +194,9 @@ _SIGNATURES = [
     ("verb_check_status", ctypes.c_int, [_P]),
     ("verb_field_update", ctypes.c_int, [_P, _P, _P, ctypes.c_size_t, ctypes.POINTER(FieldUpdateParams)]),
     ("verb_check_stability", ctypes.c_int, [_P, ctypes.POINTER(VerbGrid), ctypes.POINTER(ctypes.c_double)]),
+    ("verb_remap_batched", ctypes.c_int,
+     [_P, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    ("verb_cubic1d", ctypes.c_int, [_P, ctypes.c_int, _P, _P, _P]),
     ("verb_local_diffusion_gmres", ctypes.c_int,
      [_P, ctypes.POINTER(VerbGrid), ctypes.c_void_p, ctypes.POINTER(GmresParams), ctypes.POINTER(GmresStats)]),
     ("verb_local_diffusion_gmres_x0", ctypes.c_int,
@@ -466,3 +469,22 @@ def tridag(a, b, c, r, ctx=None):
     u = np.empty_like(r)
     ctx.check(lib().verb_tridag_batched(ctx.handle, _ptr(a), _ptr(b), _ptr(c), _ptr(r), _ptr(u), n, lines))
     return u
+
+
+def Cubic1D(old_grid, values, new_grid, ctx=None):
+    """Cubic1D (src/Interpolation.cpp:25-51): values on old_grid interpolated
+    to new_grid (same size) by the monotone cubic Hermite interpolator over
+    the points (0, 0) and (old_grid[i], values[i]). Like the reference,
+    `values` (a float64 numpy array) is replaced in place; it is also
+    returned."""
+    ctx = ctx or default_context()
+    xo = np.ascontiguousarray(old_grid, dtype=np.float64)
+    xn = np.ascontiguousarray(new_grid, dtype=np.float64)
+    if xo.size != xn.size:
+        raise ValueError("Only interpolation to the same grid size is supported.")
+    v = values if (isinstance(values, np.ndarray) and values.dtype == np.float64 and values.flags.c_contiguous) \
+        else np.ascontiguousarray(values, dtype=np.float64)
+    if v.size != xo.size:
+        raise ValueError("values and grid sizes differ")
+    ctx.check(lib().verb_cubic1d(ctx.handle, xo.size, _ptr(xo), _ptr(v), _ptr(xn)))
+    return v

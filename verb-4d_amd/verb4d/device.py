@@ -118,6 +118,18 @@ class DeviceGrid:
         self.ctx.check(lib().verb_radial_diffusion_batched(self.ctx.handle, ctypes.byref(self.grid),
                                                            self.psd.data_ptr()))
 
+    def remap(self, lstar_old, lstar_new):
+        """verb_remap_batched: the PSD interpolated along R from lstar_old to
+        lstar_new (device tensors [P][R][V][K], e.g. from torch's stream:
+        finished before the library reads them)."""
+        import torch
+        w = self.w
+        for t in (lstar_old, lstar_new):
+            assert t.is_cuda and t.dtype == torch.float64 and t.is_contiguous() and t.numel() == self.psd.numel()
+        torch.cuda.current_stream().synchronize()
+        self.ctx.check(lib().verb_remap_batched(self.ctx.handle, self.psd.data_ptr(), lstar_old.data_ptr(),
+                                                lstar_new.data_ptr(), w.nP, w.nR, w.nV, w.nK))
+
     def local(self, use_cache=False, recompute_lu=True):
         lu = self.lu_cache.data_ptr() if use_cache else None
         piv = self.ipiv_cache.data_ptr() if use_cache else None

@@ -1,7 +1,8 @@
 """The reference's time loop around the three operators, driven on the
 device-resident grid (src/VERB4D_Solver.cpp:539-1282):
 
-  clamp -> coefficient updates -> boundary updates -> stability check ->
+  remap (when L* changes) -> clamp -> coefficient updates -> boundary
+  updates -> stability check ->
   convection (-> V boundaries from the PSD) -> clamp -> radial -> clamp ->
   local (LU factors recomputed when a local coefficient changed) ->
   clamp or negative-point count
@@ -30,13 +31,15 @@ class UpdatableField:
     limits: {axis: [(time, from, to)]} with axis in 0..3 of `axes` (the
             Q*_from / Q*_to queues, :715-800)
     axes:   names of the DeviceGrid fields holding each point's coordinate
-            along each axis (4-D fields: Pg, Rg, Vg, Kg)."""
+            along each axis (4-D fields: Pg, Rg, Vg, Kg)
+    tensor: the device array to update when it is not one of the grid's
+            fields (L* for the remapping)."""
 
-    def __init__(self, dg, name, files=(), scales=(), limits=None, axes=("Pg", "Rg", "Vg", "Kg")):
+    def __init__(self, dg, name, files=(), scales=(), limits=None, axes=("Pg", "Rg", "Vg", "Kg"), tensor=None):
         import torch
         self.dg = dg
         self.name = name
-        self.cur = dg.t[name][1]
+        self.cur = dg.t[name][1] if tensor is None else tensor
         self.original = self.cur.clone()
         torch.cuda.synchronize()
         self.files = deque((float(t), np.ascontiguousarray(a, dtype=np.float64)) for t, a in files)
@@ -92,15 +95,26 @@ class TimeLoop:
             switches (src/VERB4D_Solver.cpp:621, 965-975, 1218-1221)
     lu_caching: keep the band factors between steps and recompute them only
             when losses_local, DVV, DVK or DKK changed (:1091-1096); the
-            first step always factors (the cache starts empty)."""
+            first step always factors (the cache starts empty).
+    lstar:  an UpdatableField over the L* array ([P][R][V][K], `tensor=`):
+            when it updates, the `jacobians` fields (G_local, G_radial,
+            G_conv) update too and the PSD is remapped from the previous L*
+            to the new one, before anything else of the step (:557-619)."""
 
     LOCAL = ("losses_local", "DVV", "DVK", "DKK")
 
     def __init__(self, dg, fields=None, positive_psd=False, vl_from_convection=False, vu_from_convection=False,
-                 lu_caching=False, time_first=0.0, dt_days=None):
+                 lu_caching=False, time_first=0.0, dt_days=None, lstar=None, jacobians=None):
         import torch
         self.dg = dg
         self.fields = dict(fields or {})
+        self.lstar = lstar
+        self.jacobians = dict(jacobians or {})
+        self.lstar_copy = None
+        if lstar is not None:
+            dg.ctx.synchronize()
+            self.lstar_copy = lstar.cur.clone()
+            torch.cuda.synchronize()
         self.positive_psd = positive_psd
         w = dg.w
         self.vl = torch.empty((w.nP, w.nR, w.nK), dtype=torch.float64, device="cuda") if vl_from_convection else None
@@ -115,9 +129,20 @@ class TimeLoop:
 
     def step(self, it):
         """Step `it`: returns {'updated': {name: bool}, 'negative_points': n,
-        'recompute_lu': bool, 'stability_min': value}."""
+        'recompute_lu': bool, 'stability_min': value, 'time': days,
+        'remapped': bool}."""
+        import torch
         dg = self.dg
         time = self.time_first + it * self.dt_days
+        remapped = False
+        if self.lstar is not None and self.lstar.update(time):
+            for f in self.jacobians.values():
+                f.update(time)
+            dg.remap(self.lstar_copy, self.lstar.cur)
+            dg.ctx.synchronize()
+            self.lstar_copy.copy_(self.lstar.cur)
+            torch.cuda.synchronize()
+            remapped = True
         updated = {n: f.update(time) for n, f in self.fields.items()}
         m = ctypes.c_double()
         rc = lib().verb_check_stability(dg.ctx.handle, ctypes.byref(dg.grid), ctypes.byref(m))
@@ -140,7 +165,7 @@ class TimeLoop:
         if self.vu is not None and dg.w.types["Vu"] == 1:
             dg.grid.Vu = self.vu.data_ptr()
         self.last = {"updated": updated, "negative_points": int(cnt.value), "recompute_lu": recompute,
-                     "stability_min": float(m.value), "time": time}
+                     "stability_min": float(m.value), "time": time, "remapped": remapped}
         return self.last
 
 

@@ -193,6 +193,18 @@ class Workload:
         return g
 
 
+def lstar(w, compression):
+    """Synthetic L* on the grid, [P][R][V][K]: the dipole L lowered by
+    `compression` x (L / L_max)^2 at noon (MLT angle 0), not at all at
+    midnight, more for equatorial than for field-aligned pitch angles (drift
+    shell splitting). Increasing along R while compression < 1/3;
+    compression 0 gives L* = L."""
+    P, L, A = w.f["Pg"], w.f["Rg"], w.f["Kg"]
+    Lmax = float(L.max())
+    day = 0.5 * (1.0 + np.cos(P))
+    return np.ascontiguousarray(L * (1.0 - compression * (L / Lmax) ** 2 * day * (0.6 + 0.4 * np.sin(A))))
+
+
 # Benchmark configurations (the bench's timed workload is PRODUCTION).
 def config0(seed=1, rich=False):
     """10x24x31x31 (R x MLT x E x alpha), Kp = 3, dt = 300 s, 12 steps."""
