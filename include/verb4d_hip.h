@@ -42,7 +42,12 @@ typedef enum {
     VERB_ERR_DOMAIN = -8      /* interpolation input the reference rejects: a non-finite point to
                                  evaluate or NaN setting the data's direction (the interpolator
                                  throws, MonotCubicInterpolator.cpp:178-180, 464), or a NaN grid
-                                 value (no place in std::map's order) */
+                                 value (no place in std::map's order) */,
+    VERB_ERR_IO = -9,         /* file missing, unreadable, short or not writable */
+    VERB_ERR_FORMAT = -10     /* file content the reference's readers reject or misread: a token
+                                 that is not a number, more data than the sizes ("Check grid
+                                 size", Matrix.cpp:4957-4961), a grid mismatch in a checked read
+                                 (Matrix.cpp:5390-5393) */
 } verb_status_t;
 
 /* Values of the reference's BoundaryConditionType enum class. */
@@ -468,6 +473,81 @@ int verb_local_diffusion_learned(verb_ctx* ctx, const verb_grid* g, double* psd,
 /* convection + radial + verb_local_diffusion_learned */
 int verb_step_learned(verb_ctx* ctx, const verb_grid* g, double* psd, verb_unet* net,
                       const verb_gmres_params* params, verb_gmres_stats* stats);
+
+/* ------------------------------------------------------------------------
+ * On-disk formats (host only; no device work). ndim is 3 (boundary arrays)
+ * or 4 ([P][R][V][K] fields); sizes[0..ndim) in the array's index order,
+ * arrays row-major with the last index fastest.
+ * ---------------------------------------------------------------------- */
+/* Matrix4D/3D::writeToFile(filename, info) (src/Matrix.cpp:4556-4567,
+ * 3011-3024): VARIABLES line with `name` ("f" if empty), ZONE line with the
+ * title `info` (the path if empty) and I (fastest), J, K[, L], then one
+ * value per line in "%.6e". */
+int verb_plt_write(const char* path, int ndim, const int* sizes, const double* data, const char* name,
+                   const char* info);
+/* writeToFile(filename, grids...) (src/Matrix.cpp:4833-4853, 3031-3050): every
+ * line "\t" grid coordinates "\t" value; grid_names nullable ("w","x","y","z"). */
+int verb_plt_write_grid(const char* path, int ndim, const int* sizes, const double* const* grids,
+                        const char* const* grid_names, const double* data, const char* name);
+/* The sizes of a .plt file from its ZONE line, in array index order
+ * (sizes[0] = L or K, ...); returns the number of sizes (3 or 4) or an
+ * error (the grid.plt parse of src/ReadInitialData.cpp:297-310). */
+int verb_plt_sizes(const char* path, int* sizes);
+/* readFromFile(filename, column) (src/Matrix.cpp:4906-4965, 3059-3096):
+ * the column-th token of each point line; more data than the sizes is an
+ * error. */
+int verb_plt_read(const char* path, int column, int ndim, const int* sizes, double* out);
+/* readFromFile(filename, grids...) (src/Matrix.cpp:5344-5413, 3431-3477):
+ * each line's coordinates must agree with grids[a][i] within 1e-6 in
+ * log10; the first point that does not is returned in *bad_index
+ * (nullable) with VERB_ERR_FORMAT. */
+int verb_plt_read_checked(const char* path, int ndim, const int* sizes, const double* const* grids, double* out,
+                          long long* bad_index);
+/* writeToBinaryFile / readFromBinaryFile (src/Matrix.cpp:5704-5726,
+ * 5792-5818, 3781-3838): ndim int32 sizes, then the doubles. The reader
+ * reads count(sizes) values and reports the file's own sizes (nullable),
+ * which the reference does not check either. */
+int verb_pltb_write(const char* path, int ndim, const int* sizes, const double* data);
+int verb_pltb_read(const char* path, int ndim, const int* sizes, double* out, int* file_sizes);
+/* The P, R, V, K grids of an input folder (folder ends with '/'):
+ * grid.pltb (4 int32 sizes, then the four grids) if it opens, else the
+ * four columns of grid.plt (src/ReadInitialData.cpp:253-346). With P ==
+ * NULL only sizes[4] is filled; else each grid takes `capacity` doubles. */
+int verb_grid_read(const char* folder, int* sizes, double* P, double* R, double* V, double* K, size_t capacity);
+/* Matrix4D::writeToLstFile (src/Matrix.cpp:5764-5785): appends
+ * "info<TAB>filename.plt|.pltb" (the output folder removed from the name)
+ * to output_folder + "PSD.lst". */
+int verb_psd_lst_append(const char* output_folder, const char* filename, int binary, const char* info);
+
+/* One .tab file's first line as UpdatableMatrix::readFromIniFile /
+ * readFromString split it (src/UpdatableMatrix.cpp:308-551): the data field
+ * (data_kind 0: none or "-", 1: a data file, 2: a .lst update list, by the
+ * reference's suffix test), then, while the line's stream has not ended,
+ * the scale field and up to four from/to limit pairs (a field read past
+ * trailing blanks is empty, as the reference's is). Each field is a number
+ * (verb_is_number) or the name of a "time value" queue file. */
+#define VERB_TAB_STRLEN 1024
+typedef struct {
+    int has_line;       /* 0: empty or comment first line (nothing read) */
+    int data_kind;
+    int scale_present;
+    int nlimits;        /* limit pairs read (the array is limited if > 0) */
+    char data[VERB_TAB_STRLEN];
+    char scale[VERB_TAB_STRLEN];
+    char from[4][VERB_TAB_STRLEN];
+    char to[4][VERB_TAB_STRLEN];
+} verb_tab_line;
+int verb_tab_read(const char* tab_path, verb_tab_line* out);
+/* is_number (src/UpdatableMatrix.cpp:1081-1093) and the value an istream
+ * reads from such a string (0 when none). */
+int verb_is_number(const char* s);
+double verb_parse_number(const char* s);
+/* fill_queue_from_file (src/UpdatableMatrix.cpp:349-370): "time value"
+ * lines; values as numbers (values != NULL) or file names (names != NULL,
+ * name_len bytes per entry). *count receives the number of lines with
+ * content; call with capacity 0 to size the buffers. */
+int verb_queue_read(const char* path, double* times, double* values, char* names, size_t name_len, size_t capacity,
+                    size_t* count);
 
 #ifdef __cplusplus
 }

@@ -18,6 +18,7 @@ REF_LIB = os.path.join(HERE, "_ref", "libverbref.so")
 _dp = ctypes.POINTER(ctypes.c_double)
 _i64p = ctypes.POINTER(ctypes.c_int64)
 _I = ctypes.c_int
+_ip = ctypes.POINTER(ctypes.c_int)
 _D = ctypes.c_double
 
 
@@ -117,6 +118,18 @@ def ref_lib(variant="ref"):
             "ref_step_local_cached": [ctypes.POINTER(Grid), _dp, _I, _dp, _i64p, _I, ctypes.c_long,
                                       ctypes.c_long],
             "ref_cubic1d": [_I, _dp, _dp, _dp],
+            "ref_plt4_write": [ctypes.c_char_p, _ip, _dp, ctypes.c_char_p, ctypes.c_char_p],
+            "ref_plt4_write_grid": [ctypes.c_char_p, _ip, _dp, _dp, _dp, _dp, _dp, ctypes.c_char_p],
+            "ref_plt4_read_column": [ctypes.c_char_p, _I, _ip, _dp],
+            "ref_plt4_read_checked": [ctypes.c_char_p, _ip, _dp, _dp, _dp, _dp, _dp],
+            "ref_pltb4_write": [ctypes.c_char_p, _ip, _dp],
+            "ref_pltb4_read": [ctypes.c_char_p, _ip, _dp],
+            "ref_lst_append": [ctypes.c_char_p, _I, ctypes.c_char_p, ctypes.c_char_p],
+            "ref_plt3_write": [ctypes.c_char_p, _ip, _dp, ctypes.c_char_p, ctypes.c_char_p],
+            "ref_plt3_write_grid": [ctypes.c_char_p, _ip, _dp, _dp, _dp, _dp, ctypes.c_char_p],
+            "ref_plt3_read_checked": [ctypes.c_char_p, _ip, _dp, _dp, _dp, _dp],
+            "ref_pltb3_write": [ctypes.c_char_p, _ip, _dp],
+            "ref_pltb3_read": [ctypes.c_char_p, _ip, _dp],
             "ref_remap": [_dp, _dp, _dp, _I, _I, _I, _I],
         }
         for name, args in sigs.items():

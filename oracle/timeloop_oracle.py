@@ -19,25 +19,41 @@ import oracle as O
 
 
 class UpdatableArray:
-    def __init__(self, arr, files=(), scales=(), limits=None, grids=()):
+    """files: [(t, array | path | "-")] (paths read by `loader` when due);
+    scales: [(t, s)], `scale` the coefficient before them; limits: {ax: [(t,
+    from, to)]} or limit_from / limit_to {ax: [(t, v)]}, limit_init {ax:
+    (from, to)} constants; grids: the coordinate arrays of the limited axes."""
+
+    def __init__(self, arr, files=(), scales=(), limits=None, grids=(), scale=1.0, limit_from=None, limit_to=None,
+                 limit_init=None, loader=None):
         self.original = np.array(arr, dtype=np.float64, copy=True)
         self.current = self.original.copy()
-        self.files = deque((float(t), np.asarray(a, dtype=np.float64)) for t, a in files)
+        self.files = deque((float(t), a if isinstance(a, str) else np.asarray(a, dtype=np.float64)) for t, a in files)
+        self.loader = loader
         self.scales = deque((float(t), float(s)) for t, s in scales)
-        self.limits = {ax: deque((float(t), float(f), float(to)) for t, f, to in q)
-                       for ax, q in (limits or {}).items()}
+        self.from_q = {ax: deque((float(t), float(v)) for t, v in q) for ax, q in (limit_from or {}).items()}
+        self.to_q = {ax: deque((float(t), float(v)) for t, v in q) for ax, q in (limit_to or {}).items()}
+        for ax, q in (limits or {}).items():
+            self.from_q.setdefault(ax, deque()).extend((float(t), float(f)) for t, f, _ in q)
+            self.to_q.setdefault(ax, deque()).extend((float(t), float(to)) for t, _, to in q)
         self.grids = grids
         self.last_update_time = -1.0
-        self.latest_scale = 1.0
-        self.rng = {ax: (-1e99, 1e99) for ax in range(4)}
+        self.latest_scale = float(scale)
+        self.rng = {ax: [-1e99, 1e99] for ax in range(4)}
+        for ax, (f, to) in (limit_init or {}).items():
+            self.rng[ax] = [float(f), float(to)]
+        self.limited = set(self.from_q) | set(self.to_q) | set(limit_init or {})
 
     def update(self, t):
         updated = False
         data, ut = None, None
         while self.files and t >= self.files[0][0]:
             ut, data = self.files.popleft()
-        if data is not None and t >= ut and self.last_update_time < ut:
-            self.original = data.reshape(self.original.shape).copy()
+        if data is not None and not (isinstance(data, str) and data in ("", "-")) and t >= ut \
+                and self.last_update_time < ut:
+            if isinstance(data, str):
+                data = self.loader(data)
+            self.original = np.asarray(data, dtype=np.float64).reshape(self.original.shape).copy()
             self.last_update_time = t
             updated = True
         cur = self.original.copy()
@@ -46,17 +62,15 @@ class UpdatableArray:
             updated = True
         if self.latest_scale != 1:
             cur = cur * self.latest_scale
-        for ax, q in self.limits.items():
-            while q and t >= q[0][0]:
-                _, f, to = q.popleft()
-                self.rng[ax] = (f, to)
-                updated = True
+        for qs, k in ((self.to_q, 1), (self.from_q, 0)):
+            for ax, q in qs.items():
+                while q and t >= q[0][0]:
+                    self.rng[ax][k] = q.popleft()[1]
+                    updated = True
         # MatrixLimit: zero where any limited coordinate is out of range
-        limit = any(self.rng[ax][0] > self.grids[ax].min() or self.rng[ax][1] < self.grids[ax].max()
-                    for ax in self.limits)
-        if limit:
+        if self.limited:
             out = np.zeros(cur.shape, dtype=bool)
-            for ax in self.limits:
+            for ax in self.limited:
                 q = self.grids[ax]
                 out |= (q < self.rng[ax][0]) | (q > self.rng[ax][1])
             cur = np.where(out, 0.0, cur)

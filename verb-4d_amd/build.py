@@ -29,7 +29,10 @@ SOURCES = [
     ("api.hip", []),
     ("noalloc.hip", []),
     ("unet.hip", []),
+    ("formats.cpp", []),  # host only: compiled with g++
 ]
+
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
 
 COMMON = [
     "--offload-arch=gfx950",
@@ -67,12 +70,15 @@ def build(verbose=False, incremental=False, jobs=8):
     os.makedirs(OUT_DIR, exist_ok=True)
     objs, procs = [], []
     for src, extra in SOURCES:
-        obj = os.path.join(OUT_DIR, src.replace(".hip", ".o"))
+        obj = os.path.join(OUT_DIR, os.path.splitext(src)[0] + ".o")
         objs.append(obj)
         path = os.path.join(CSRC, src)
         if incremental and not _stale(obj, path):
             continue
-        cmd = [HIPCC] + COMMON + extra + ["-c", path, "-o", obj]
+        if src.endswith(".cpp"):
+            cmd = ["g++"] + HOST_FLAGS + extra + ["-c", path, "-o", obj]
+        else:
+            cmd = [HIPCC] + COMMON + extra + ["-c", path, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         while len([p for p in procs if p[0].poll() is None]) >= jobs:

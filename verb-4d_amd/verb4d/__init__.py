@@ -99,6 +99,17 @@ class MixedStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+TAB_STRLEN = 1024
+
+
+class TabLine(ctypes.Structure):
+    """verb_tab_line: the first line of a .tab file, split as the reference
+    splits it."""
+    _fields_ = [("has_line", ctypes.c_int), ("data_kind", ctypes.c_int), ("scale_present", ctypes.c_int),
+                ("nlimits", ctypes.c_int), ("data", ctypes.c_char * TAB_STRLEN), ("scale", ctypes.c_char * TAB_STRLEN),
+                ("from_", (ctypes.c_char * TAB_STRLEN) * 4), ("to", (ctypes.c_char * TAB_STRLEN) * 4)]
+
+
 class Tuning(ctypes.Structure):
     """ctypes mirror of ``verb_tuning``."""
 
@@ -197,6 +208,23 @@ _SIGNATURES = [
     ("verb_remap_batched", ctypes.c_int,
      [_P, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     ("verb_cubic1d", ctypes.c_int, [_P, ctypes.c_int, _P, _P, _P]),
+    ("verb_plt_write", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, _P, _P, ctypes.c_char_p, ctypes.c_char_p]),
+    ("verb_plt_write_grid", ctypes.c_int,
+     [ctypes.c_char_p, ctypes.c_int, _P, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_char_p), _P,
+      ctypes.c_char_p]),
+    ("verb_plt_sizes", ctypes.c_int, [ctypes.c_char_p, _P]),
+    ("verb_plt_read", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _P, _P]),
+    ("verb_plt_read_checked", ctypes.c_int,
+     [ctypes.c_char_p, ctypes.c_int, _P, ctypes.POINTER(ctypes.c_void_p), _P, ctypes.POINTER(ctypes.c_longlong)]),
+    ("verb_pltb_write", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, _P, _P]),
+    ("verb_pltb_read", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, _P, _P, _P]),
+    ("verb_grid_read", ctypes.c_int, [ctypes.c_char_p, _P, _P, _P, _P, _P, ctypes.c_size_t]),
+    ("verb_psd_lst_append", ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p]),
+    ("verb_tab_read", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(TabLine)]),
+    ("verb_is_number", ctypes.c_int, [ctypes.c_char_p]),
+    ("verb_parse_number", ctypes.c_double, [ctypes.c_char_p]),
+    ("verb_queue_read", ctypes.c_int,
+     [ctypes.c_char_p, _P, _P, _P, ctypes.c_size_t, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
     ("verb_local_diffusion_gmres", ctypes.c_int,
      [_P, ctypes.POINTER(VerbGrid), ctypes.c_void_p, ctypes.POINTER(GmresParams), ctypes.POINTER(GmresStats)]),
     ("verb_local_diffusion_gmres_x0", ctypes.c_int,

@@ -24,33 +24,55 @@ from . import FieldUpdateParams, StepOptions, VerbError, lib
 class UpdatableField:
     """One device field of a DeviceGrid with UpdatableMatrix semantics.
 
-    files:  [(time, array)] replacement originals (the .lst queue: the last
-            entry due at the current time is loaded, once per update time,
-            src/UpdatableMatrix.cpp:636-663)
-    scales: [(time, coefficient)] (the scale queue, :674-713)
-    limits: {axis: [(time, from, to)]} with axis in 0..3 of `axes` (the
-            Q*_from / Q*_to queues, :715-800)
+    files:  [(time, array | path | "-")] replacement originals (the .lst
+            queue: the last entry due at the current time is loaded, once
+            per update time, src/UpdatableMatrix.cpp:636-663; a path is read
+            with `loader` when it comes due, "-" loads nothing)
+    scales: [(time, coefficient)] (the scale queue, :674-713); `scale` is the
+            coefficient before the first entry (a constant scale field)
+    limits: {axis: [(time, from, to)]}, or separate `limit_from` /
+            `limit_to` {axis: [(time, value)]} queues, with axis in 0..3 of
+            `axes` (the Q*_from / Q*_to queues, :715-800); `limit_init`
+            {axis: (from, to)} are constant limits
     axes:   names of the DeviceGrid fields holding each point's coordinate
             along each axis (4-D fields: Pg, Rg, Vg, Kg)
     tensor: the device array to update when it is not one of the grid's
             fields (L* for the remapping)."""
 
-    def __init__(self, dg, name, files=(), scales=(), limits=None, axes=("Pg", "Rg", "Vg", "Kg"), tensor=None):
+    def __init__(self, dg, name, files=(), scales=(), limits=None, axes=("Pg", "Rg", "Vg", "Kg"), tensor=None,
+                 scale=1.0, limit_from=None, limit_to=None, limit_init=None, loader=None):
         import torch
         self.dg = dg
         self.name = name
         self.cur = dg.t[name][1] if tensor is None else tensor
         self.original = self.cur.clone()
         torch.cuda.synchronize()
-        self.files = deque((float(t), np.ascontiguousarray(a, dtype=np.float64)) for t, a in files)
-        self.scales = deque((float(t), float(s)) for t, s in scales)
-        self.limit_q = {}
+        self.files = deque((float(t), a if isinstance(a, str) else np.ascontiguousarray(a, dtype=np.float64))
+                           for t, a in files)
+        self.loader = loader
+        self.scales = deque((float(t), float(v)) for t, v in scales)
+        self.from_q = {ax: deque((float(t), float(v)) for t, v in q) for ax, q in (limit_from or {}).items()}
+        self.to_q = {ax: deque((float(t), float(v)) for t, v in q) for ax, q in (limit_to or {}).items()}
         for ax, q in (limits or {}).items():
-            self.limit_q[ax] = deque((float(t), float(f), float(to)) for t, f, to in q)
+            self.from_q.setdefault(ax, deque()).extend((float(t), float(f)) for t, f, _ in q)
+            self.to_q.setdefault(ax, deque()).extend((float(t), float(to)) for t, _, to in q)
         self.axes = axes
         self.last_update_time = -1.0
-        self.scale = 1.0
-        self.range = {ax: (-1e99, 1e99) for ax in range(4)}
+        self.scale = float(scale)
+        self.range = {ax: [-1e99, 1e99] for ax in range(4)}
+        for ax, (f, to) in (limit_init or {}).items():
+            self.range[ax] = [float(f), float(to)]
+        self.limited = set(self.from_q) | set(self.to_q) | set(limit_init or {})
+
+    @classmethod
+    def from_spec(cls, dg, name, spec, grids, axes=("Pg", "Rg", "Vg", "Kg"), tensor=None):
+        """From verb4d.formats.field_spec (a .tab line): .lst entries are
+        read, grid-checked against `grids` (host arrays), when due."""
+        from . import formats
+        f = cls(dg, name, files=spec["files"], scales=spec["scales"], axes=axes, tensor=tensor,
+                scale=spec["scale"], limit_from=spec["limit_from"], limit_to=spec["limit_to"],
+                limit_init=spec["limit_init"], loader=lambda p: formats.read_any(p, grids))
+        return f
 
     def update(self, time):
         """Recomputes the device array for `time`; True when something was
@@ -60,26 +82,30 @@ class UpdatableField:
         data, utime = None, None
         while self.files and time >= self.files[0][0]:
             utime, data = self.files.popleft()
-        if data is not None and time >= utime and self.last_update_time < utime:
+        if data is not None and not (isinstance(data, str) and data in ("", "-")) and time >= utime \
+                and self.last_update_time < utime:
+            if isinstance(data, str):
+                data = self.loader(data)
             self.dg.ctx.synchronize()
-            self.original.copy_(torch.from_numpy(data.reshape(self.original.shape)))
+            self.original.copy_(torch.from_numpy(np.ascontiguousarray(data, dtype=np.float64).reshape(
+                self.original.shape)))
             torch.cuda.synchronize()
             self.last_update_time = time
             updated = True
         while self.scales and time >= self.scales[0][0]:
             self.scale = self.scales.popleft()[1]
             updated = True
-        for ax, q in self.limit_q.items():
-            while q and time >= q[0][0]:
-                _, f, to = q.popleft()
-                self.range[ax] = (f, to)
-                updated = True
+        for qs, k in ((self.to_q, 1), (self.from_q, 0)):
+            for ax, q in qs.items():
+                while q and time >= q[0][0]:
+                    self.range[ax][k] = q.popleft()[1]
+                    updated = True
         p = FieldUpdateParams()
         p.scale = self.scale
         for ax in range(4):
             f, to = self.range[ax]
             p.from_[ax], p.to[ax] = f, to
-            p.grid[ax] = self.dg.t[self.axes[ax]][1].data_ptr() if (ax in self.limit_q and ax < len(self.axes)) \
+            p.grid[ax] = self.dg.t[self.axes[ax]][1].data_ptr() if (ax in self.limited and ax < len(self.axes)) \
                 else None
         self.dg.ctx.check(lib().verb_field_update(self.dg.ctx.handle, self.cur.data_ptr(), self.original.data_ptr(),
                                                   self.cur.numel(), ctypes.byref(p)))
