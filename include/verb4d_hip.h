@@ -44,10 +44,12 @@ typedef enum {
                                  throws, MonotCubicInterpolator.cpp:178-180, 464), or a NaN grid
                                  value (no place in std::map's order) */,
     VERB_ERR_IO = -9,         /* file missing, unreadable, short or not writable */
-    VERB_ERR_FORMAT = -10     /* file content the reference's readers reject or misread: a token
+    VERB_ERR_FORMAT = -10,    /* file content the reference's readers reject or misread: a token
                                  that is not a number, more data than the sizes ("Check grid
                                  size", Matrix.cpp:4957-4961), a grid mismatch in a checked read
                                  (Matrix.cpp:5390-5393) */
+    VERB_ERR_SINGULAR = -11   /* a dense solve met a zero pivot (trans_solve's dgesv info > 0,
+                                 MatrixOperations.cpp:779-786, exits) */
 } verb_status_t;
 
 /* Values of the reference's BoundaryConditionType enum class. */
@@ -473,6 +475,30 @@ int verb_local_diffusion_learned(verb_ctx* ctx, const verb_grid* g, double* psd,
 /* convection + radial + verb_local_diffusion_learned */
 int verb_step_learned(verb_ctx* ctx, const verb_grid* g, double* psd, verb_unet* net,
                       const verb_gmres_params* params, verb_gmres_stats* stats);
+
+/* Data assimilation along L: the Kalman update that
+ * DataAssimilationManagerDiffusion::assimilate runs on every (V,K) line of
+ * one P slice (src/DataAssimilationDiffusion.cpp:70-249, runKalmanFilter
+ * src/DataAssimilation.cpp:18-73). Per line of n = nR points (3 <= n <= 32):
+ * F = B A^-1 from the line's implicit 1-D diffusion matrices over g's L
+ * grid, DLL and G_radial with zero sources and losses (Ll_type/Lu_type
+ * boundaries, time step params->dt); covariance = 5 model_error I when the
+ * line's is all zero; Pf = F Pa F^T + model_error I; H selects the line's
+ * finite observations; K = Pf H^T (H Pf H^T + observation_error I)^-1;
+ * psd += K (obs - H psd); Pa = Pf - K H Pf. psd and obs are device arrays
+ * [P][R][V][K] (obs non-finite = not observed); covariance [V][K][n][n]
+ * (device, in/out, zero before the first call); model_matrix [V][K][n][n]
+ * (device, nullable) receives F. Asynchronous; VERB_ERR_SINGULAR (a zero
+ * pivot) and VERB_ERR_DOMAIN (a NaN analysis, where the reference throws)
+ * are reported by verb_check_status. */
+typedef struct {
+    double dt;
+    double model_error;
+    double observation_error;
+    int p_index;
+} verb_da_params;
+int verb_da_radial(verb_ctx* ctx, const verb_grid* g, double* psd, const double* obs, double* covariance,
+                   double* model_matrix, const verb_da_params* params);
 
 /* ------------------------------------------------------------------------
  * On-disk formats (host only; no device work). ndim is 3 (boundary arrays)

@@ -3,9 +3,10 @@ reference's own hot-path code (oracle/_ref/libverbref.so, compiled by
 oracle/Makefile from the PRBEM/VERB-4D sources) on small seeded inputs.
 
 Run in a container that holds the reference tree:
-    make -C oracle && python tests/golden/make_golden.py [options|remap]
+    make -C oracle && python tests/golden/make_golden.py [options|remap|da]
 ("options" writes only the fixtures of the reference's build options,
-"remap" only those of the adiabatic remapping).
+"remap" only those of the adiabatic remapping, "da" only those of the data
+assimilation's Kalman update).
 The fixtures are data only (inputs and the reference's outputs).
 """
 import os
@@ -254,7 +255,63 @@ def remap_fixtures():
     save("remap_grid.npz", dict(nP=4, nR=10, nV=21, nK=17, seed=3, c_old=0.05, c_new=0.12, out=out))
 
 
+def da_cases(seed):
+    """Kalman updates of single lines by the reference's own matrix
+    operations (oracle/ref_da.cpp): config-0 and production radial lines,
+    observations at some points (NaN elsewhere), none, and all; each line
+    assimilated twice, the second time from the first's covariance."""
+    import ctypes
+    rng = np.random.default_rng(seed)
+    dp = ctypes.POINTER(ctypes.c_double)
+    L = O.ref_lib()
+    L.ref_da_line.argtypes = [ctypes.c_int, dp, dp, dp, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                              ctypes.c_double, dp, dp, dp, dp, dp, ctypes.c_double, ctypes.c_double]
+    p = lambda a: a.ctypes.data_as(dp)  # noqa: E731
+    cases = []
+    for w, lines in ((W.config0(seed=2, rich=True), [(3, 5, 7), (0, 20, 13), (11, 30, 0)]),
+                     (W.config1(seed=1001), [(4, 50, 45), (17, 100, 90)])):
+        for k, (iP, iV, iK) in enumerate(lines):
+            n = w.nR
+            R = np.ascontiguousarray(w.f["Lg"][iP, :, iV, iK])
+            D = np.ascontiguousarray(w.f["DLL"][iP, :, iV, iK])
+            G = np.ascontiguousarray(w.f["G_radial"][iP, :, iV, iK])
+            x = np.ascontiguousarray(w.psd0[iP, :, iV, iK])
+            for kind in ("some", "none", "all"):
+                obs = x * (1 + 0.2 * rng.standard_normal(n))
+                if kind == "some":
+                    obs[rng.random(n) < 0.6] = np.nan
+                elif kind == "none":
+                    obs[:] = np.nan
+                Pa = np.zeros((n, n))
+                xs, Ps, Fs, Pfs = [], [], [], []
+                xx = x.copy()
+                for rep in range(2):
+                    F = np.zeros((n, n))
+                    Pf = np.zeros((n, n))
+                    st = L.ref_da_line(n, p(R), p(D), p(G), float(x[0]), float(x[-1]), w.types["Ll"], w.types["Lu"],
+                                       3600.0, p(obs), p(xx), p(Pa), p(F), p(Pf), 1e-2 * float(np.abs(x).max()) ** 2,
+                                       4e-2 * float(np.abs(x).max()) ** 2)
+                    assert st == 0
+                    xs.append(xx.copy())
+                    Ps.append(Pa.copy())
+                    Fs.append(F)
+                    Pfs.append(Pf)
+                cases.append(dict(R=R, D=D, G=G, x0=x, obs=obs, lt=w.types["Ll"], ut=w.types["Lu"], dt=3600.0,
+                                  me=1e-2 * float(np.abs(x).max()) ** 2, oe=4e-2 * float(np.abs(x).max()) ** 2,
+                                  x1=xs[0], x2=xs[1], P1=Ps[0], P2=Ps[1], F=Fs[0], Pf1=Pfs[0], Pf2=Pfs[1]))
+    flat = {"count": len(cases)}
+    for i, c in enumerate(cases):
+        for k, v in c.items():
+            flat["%d_%s" % (i, k)] = v
+    return flat
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "da":
+        if not O.ref_available():
+            O.build()
+        save("da_lines.npz", da_cases(31))
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "options":
         option_fixtures()
         return

@@ -26,6 +26,7 @@ SOURCES = [
     ("gmres.hip", []),
     ("driver.hip", ["-ffp-contract=off"]),
     ("remap.hip", ["-ffp-contract=off"]),
+    ("da.hip", []),
     ("api.hip", []),
     ("noalloc.hip", []),
     ("unet.hip", []),

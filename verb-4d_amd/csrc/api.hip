@@ -99,8 +99,10 @@ int finish(verb_ctx* c) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (st[0] == VERB_ERR_COURANT) return set_err(c, VERB_ERR_COURANT, "max(CourNum) > 1, calculation can't be performed");
     if (st[0] == VERB_ERR_BC) return set_err(c, VERB_ERR_BC, "unknown boundary type");
+    if (st[0] == VERB_ERR_SINGULAR) return set_err(c, VERB_ERR_SINGULAR, "trans_solve: zero pivot");
     if (st[0] == VERB_ERR_DOMAIN)
-        return set_err(c, VERB_ERR_DOMAIN, "MonotCubicInterpolator: non-finite input the interpolator rejects");
+        return set_err(c, VERB_ERR_DOMAIN, "non-finite input or result the reference rejects (interpolation input, "
+                                           "a NaN Kalman analysis)");
     if (st[0] != 0) return set_err(c, st[0], "device error");
     if (st[1] != 0x7fffffff) return set_err(c, st[1], "Error computing LU decomposition (zero pivot)");
     return VERB_OK;
@@ -849,6 +851,19 @@ int verb_remap_batched(verb_ctx* c, double* psd, const double* lstar_old, const 
     std::lock_guard<std::mutex> lk(c->mu);
     int rc = verb::launch_remap(psd, lstar_old, lstar_new, nP, nR, nV, nK, c->status, c->stream);
     return rc ? set_err(c, rc, "remap launch rejected") : VERB_OK;
+}
+
+int verb_da_radial(verb_ctx* c, const verb_grid* g, double* psd, const double* obs, double* covariance,
+                   double* model_matrix, const verb_da_params* p) {
+    if (!c || !grid_ok(g) || !psd || !obs || !covariance || !p || !g->Lg || !g->DLL || !g->G_radial)
+        return VERB_ERR_ARG;
+    if (!(p->dt > 0) || p->p_index < 0 || p->p_index >= g->nP) return VERB_ERR_ARG;
+    if (!bc_ok(g->Ll_type) || !bc_ok(g->Lu_type) || g->Ll_type == VERB_BC_PERIODIC || g->Lu_type == VERB_BC_PERIODIC)
+        return VERB_ERR_BC;
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = verb::launch_da_radial(*g, psd, obs, covariance, model_matrix, p->p_index, p->dt, p->model_error,
+                                    p->observation_error, c->status, c->stream);
+    return rc ? set_err(c, rc, "data assimilation launch rejected (3 <= nR <= 32)") : VERB_OK;
 }
 
 int verb_cubic1d(verb_ctx* c, int n, const double* x_old, double* values, const double* x_new) {

@@ -151,6 +151,10 @@ int launch_field_update(double* dst, const double* src, size_t n, const verb_fie
 constexpr int kStabilityParts = 1024;
 int launch_stability_min(const verb_grid& g, double* part, double* out, hipStream_t stream);
 
+// da.hip: the Kalman update along L on every (V,K) line of P slice iP (3 <= nR <= 32)
+int launch_da_radial(const verb_grid& g, double* psd, const double* obs, double* Pa, double* F_out, int iP, double dt,
+                     double model_error, double observation_error, int* status, hipStream_t stream);
+
 // remap.hip: Cubic1D on every (P,V,K) line (nR <= 105)
 int launch_remap(double* psd, const double* lold, const double* lnew, int nP, int nR, int nV, int nK, int* status,
                  hipStream_t stream);

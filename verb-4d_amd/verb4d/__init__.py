@@ -110,6 +110,11 @@ class TabLine(ctypes.Structure):
                 ("from_", (ctypes.c_char * TAB_STRLEN) * 4), ("to", (ctypes.c_char * TAB_STRLEN) * 4)]
 
 
+class DaParams(ctypes.Structure):
+    _fields_ = [("dt", ctypes.c_double), ("model_error", ctypes.c_double), ("observation_error", ctypes.c_double),
+                ("p_index", ctypes.c_int)]
+
+
 class Tuning(ctypes.Structure):
     """ctypes mirror of ``verb_tuning``."""
 
@@ -208,6 +213,8 @@ _SIGNATURES = [
     ("verb_remap_batched", ctypes.c_int,
      [_P, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     ("verb_cubic1d", ctypes.c_int, [_P, ctypes.c_int, _P, _P, _P]),
+    ("verb_da_radial", ctypes.c_int,
+     [_P, ctypes.POINTER(VerbGrid), _P, _P, _P, _P, ctypes.POINTER(DaParams)]),
     ("verb_plt_write", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, _P, _P, ctypes.c_char_p, ctypes.c_char_p]),
     ("verb_plt_write_grid", ctypes.c_int,
      [ctypes.c_char_p, ctypes.c_int, _P, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_char_p), _P,

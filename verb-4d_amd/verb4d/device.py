@@ -118,6 +118,19 @@ class DeviceGrid:
         self.ctx.check(lib().verb_radial_diffusion_batched(self.ctx.handle, ctypes.byref(self.grid),
                                                            self.psd.data_ptr()))
 
+    def da_radial(self, obs, covariance, dt, model_error, observation_error, p_index=0, model_matrix=None):
+        """verb_da_radial: the Kalman update along L of every (V,K) line of P
+        slice p_index; obs [P][R][V][K] and covariance [V][K][R][R] device
+        tensors (covariance in/out, zero before the first call)."""
+        import torch
+        from . import DaParams
+        p = DaParams(float(dt), float(model_error), float(observation_error), int(p_index))
+        torch.cuda.current_stream().synchronize()
+        self.ctx.check(lib().verb_da_radial(self.ctx.handle, ctypes.byref(self.grid), self.psd.data_ptr(),
+                                            obs.data_ptr(), covariance.data_ptr(),
+                                            model_matrix.data_ptr() if model_matrix is not None else None,
+                                            ctypes.byref(p)))
+
     def remap(self, lstar_old, lstar_new):
         """verb_remap_batched: the PSD interpolated along R from lstar_old to
         lstar_new (device tensors [P][R][V][K], e.g. from torch's stream:
