@@ -408,9 +408,12 @@ __global__ __launch_bounds__(kThreads) void convection_kernel(const verb_grid g,
     double* F = lds + 2 * npt;
     double* FCP = lds + 3 * npt;
     double* FCR = lds + 4 * npt;
-    double* dtq = lds + 5 * npt;            // [TQ]
-    double* red = dtq + TQ;                 // [2*kThreads]
-    int* ninP = (int*)(red + 2 * kThreads); // [nR*TQ]
+    // the velocity reduction [2*kThreads] is done before F, FCP and FCR are
+    // first written (a barrier lies between), so it shares their space
+    double* red = F;
+    const int red_extra = 2 * kThreads > 3 * npt ? 2 * kThreads - 3 * npt : 0;
+    double* dtq = lds + 5 * npt + red_extra;  // [TQ]
+    int* ninP = (int*)(dtq + TQ);           // [nR*TQ]
     int* ninR = ninP + g.nR * TQ;           // [nP*TQ]
     int* nsteps = ninR + g.nP * TQ;         // [TQ]
     int* active = nsteps + TQ;              // [TQ]
@@ -622,7 +625,8 @@ __global__ __launch_bounds__(kThreads) void convection_kernel(const verb_grid g,
 
 size_t convection_lds_bytes(int nP, int nR, int TQ) {
     const size_t npt = (size_t)nP * nR * TQ;
-    size_t b = (5 * npt + TQ + 2 * kThreads) * sizeof(double);
+    const size_t red_extra = 2 * (size_t)kThreads > 3 * npt ? 2 * (size_t)kThreads - 3 * npt : 0;
+    size_t b = (5 * npt + red_extra + TQ) * sizeof(double);
     const size_t nint = (size_t)(nR + nP) * TQ + 2 * TQ;
     b += (nint + (nint & 1)) * sizeof(int);
     b += (size_t)(nR + nP) * TQ * 4 * sizeof(double);
