@@ -391,6 +391,7 @@ def main():
     ap.add_argument("--no-cached", action="store_true",
                     help="skip timing steps that reuse cached LU factors (the reference's LU_CACHING mode)")
     ap.add_argument("--no-remap", action="store_true", help="skip the adiabatic remapping line")
+    ap.add_argument("--no-da", action="store_true", help="skip the data assimilation line")
     ap.add_argument("--no-gmres", action="store_true",
                     help="skip timing steps whose local solve is the matrix-free GMRES path")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
@@ -411,6 +412,8 @@ def main():
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported (the multi-GPU "
               "sharding is a design note, not an implemented path)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+
+    import ctypes
 
     import numpy as np
     import torch
@@ -551,6 +554,42 @@ def main():
                  "ms": round(ms, 4), "achieved": round(gbs, 2), "unit": "GB/s", "frac": round(gbs / PEAK_HBM_GBS, 4),
                  "bytes_per_point": 32}
         del lo, ln
+
+    da = None
+    if not args.no_da:
+        # the Kalman update along L on every (V,K) line of one MLT slice: per
+        # line (n = 29 points, m observed) F = B A^-1 (LU of A^T, n solves),
+        # F Pa F^T, the gain (LU of S^T, n solves) and K H Pf
+        rng = np.random.default_rng(5)
+        n = w.nR
+        obs = np.full(w.shape, np.nan)
+        keep = rng.random(w.shape[1:]) < 0.4
+        obs[0] = np.where(keep, w.psd0[0] * (1 + 0.2 * rng.standard_normal(w.shape[1:])), np.nan)
+        m = keep.sum(axis=0)  # observed points per line
+        tobs = torch.from_numpy(obs).cuda()
+        cov = torch.zeros((w.nV, w.nK, n, n), dtype=torch.float64, device="cuda")
+        sc = float(np.abs(w.psd0[0]).max())
+        dg.set_psd(w.psd0)
+        dg.da_radial(tobs, cov, 3600.0, 1e-2 * sc ** 2, 4e-2 * sc ** 2)
+        dg.check_status()
+        reps = max(10, args.steps)
+        from verb4d import DaParams
+        prm = DaParams(3600.0, 1e-2 * sc ** 2, 4e-2 * sc ** 2, 0)
+        ev0.record(ext)
+        for _ in range(reps):
+            ctx.check(verb4d.lib().verb_da_radial(ctx.handle, ctypes.byref(dg.grid), dg.psd.data_ptr(), tobs.data_ptr(),
+                                                  cov.data_ptr(), None, ctypes.byref(prm)))
+        ev1.record(ext)
+        ctx.synchronize()
+        torch.cuda.synchronize()
+        dg.check_status()
+        ms = ev0.elapsed_time(ev1) / reps
+        flops = float(np.sum(2 * n ** 3 / 3 + 2 * n ** 3 + 2 * n ** 3 + 2 * n ** 3 + 2 * m ** 3 / 3 + 2 * m ** 2 * n
+                             + 2 * n * n * m + 2 * n * m))
+        da = {"workload": "production grid, MLT slice 0: 9191 (E, alpha) lines of 29 points, 40 % observed",
+              "ms": round(ms, 4), "gflop": round(flops / 1e9, 3), "achieved": round(flops / (ms * 1e-3) / 1e12, 3),
+              "unit": "TFLOP/s", "frac": round(flops / (ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS, 4)}
+        del tobs, cov
 
     gm = None
     gm8 = None
@@ -697,6 +736,8 @@ def main():
         out["uneven_extents"] = uneven
     if remap is not None:
         out["remap"] = remap
+    if da is not None:
+        out["data_assimilation"] = da
     if base:
         out["speedup_vs_cpu"] = round(value / base["value"], 2)
     print(json.dumps(out), flush=True)

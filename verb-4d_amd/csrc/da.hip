@@ -80,19 +80,24 @@ __device__ inline void gemm_lds(double* C, const double* A, const double* B, int
 }
 
 // In place LU with partial pivoting of the m x m matrix M (dgetf2's pivot
-// choice: the first largest |value| in the column), then X (rhs rows x m,
-// each row one right-hand side b^T) replaced by the solutions of
-// M y = b: X[r][:] = M^-1 X[r][:]^T. Returns false on a zero pivot.
+// choice: the first largest |value| in the column), then X (rhs rows of
+// length m, each row one right-hand side b^T) replaced by the solutions of
+// M y = b. m, rhs <= 32. Lane (c, h) = (lane % 32, lane / 32) works on
+// column / right-hand side c and every other row from h, so each
+// elimination step is one short loop per lane with no index division.
+// Returns false on a zero pivot.
 __device__ inline bool lu_solve_rows(double* M, double* X, int m, int rhs, int lane, int* piv) {
+    const int c = lane & 31, h = lane >> 5;
     bool ok = true;
     for (int j = 0; j < m; j++) {
-        // pivot search over rows j..m-1 of column j (lane-parallel, then
-        // the first index of the maximum)
+        // pivot search over rows j..m-1 of column j, then the first index of
+        // the maximum
         double best = -1.0;
         int bi = m;
-        for (int i = j + lane; i < m; i += 64) {
-            const double v = fabs(M[i * kDaLd + j]);
-            if (v > best) { best = v; bi = i; }
+        const int i0 = j + lane;
+        if (i0 < m) {
+            best = fabs(M[i0 * kDaLd + j]);
+            bi = i0;
         }
         for (int o = 32; o >= 1; o >>= 1) {
             const double ob = __shfl_xor(best, o);
@@ -102,51 +107,54 @@ __device__ inline bool lu_solve_rows(double* M, double* X, int m, int rhs, int l
         const int p = bi;
         if (lane == 0) piv[j] = p;
         if (!(best > 0.0)) ok = false;
-        __builtin_amdgcn_wave_barrier();
-        if (p != j && p < m) {
-            for (int c = lane; c < m; c += 64) {
-                const double t = M[j * kDaLd + c];
-                M[j * kDaLd + c] = M[p * kDaLd + c];
-                M[p * kDaLd + c] = t;
-            }
+        if (p != j && p < m && h == 0 && c < m) {
+            const double t = M[j * kDaLd + c];
+            M[j * kDaLd + c] = M[p * kDaLd + c];
+            M[p * kDaLd + c] = t;
         }
         __builtin_amdgcn_wave_barrier();
         const double d = M[j * kDaLd + j];
-        if (d != 0.0) {
-            const double rinv = 1.0 / d;
-            for (int i = j + 1 + lane; i < m; i += 64) M[i * kDaLd + j] *= rinv;
-        }
+        const double rinv = d != 0.0 ? 1.0 / d : 0.0;
+        if (h == 0 && j + 1 + c < m && d != 0.0) M[(j + 1 + c) * kDaLd + j] *= rinv;
         __builtin_amdgcn_wave_barrier();
-        // rank-1 update of the trailing block, one lane per (row, column)
-        const int w = m - j - 1;
-        for (int t = lane; t < w * w; t += 64) {
-            const int i = j + 1 + t / w, c = j + 1 + t % w;
-            M[i * kDaLd + c] -= M[i * kDaLd + j] * M[j * kDaLd + c];
+        // rank-1 update of the trailing block: lane (c, h) takes column j+1+c,
+        // rows j+1+h, j+3+h, ...
+        const int col = j + 1 + c;
+        if (col < m) {
+            const double u = M[j * kDaLd + col];
+            for (int i = j + 1 + h; i < m; i += 2) M[i * kDaLd + col] -= M[i * kDaLd + j] * u;
         }
         __builtin_amdgcn_wave_barrier();
     }
-    // one lane per right-hand side: interchanges, forward and back substitution
-    for (int r = lane; r < rhs; r += 64) {
-        double* x = X + r * kDaLd;
-        for (int j = 0; j < m; j++) {
-            const int p = piv[j];
-            if (p != j) {
-                const double t = x[j];
-                x[j] = x[p];
-                x[p] = t;
-            }
-        }
-        for (int j = 0; j < m; j++) {
-            const double v = x[j];
-            for (int i = j + 1; i < m; i++) x[i] -= M[i * kDaLd + j] * v;
-        }
-        for (int j = m - 1; j >= 0; j--) {
-            double v = x[j] / M[j * kDaLd + j];
-            x[j] = v;
-            for (int i = 0; i < j; i++) x[i] -= M[i * kDaLd + j] * v;
+    // right-hand side c: interchanges, then forward and back substitution,
+    // rows split over the two lane halves
+    double* x = X + c * kDaLd;
+    const bool mine = c < rhs;
+    for (int j = 0; j < m; j++) {
+        const int p = piv[j];
+        if (mine && h == 0 && p != j) {
+            const double t = x[j];
+            x[j] = x[p];
+            x[p] = t;
         }
     }
     __builtin_amdgcn_wave_barrier();
+    for (int j = 0; j < m; j++) {
+        if (mine) {
+            const double v = x[j];
+            for (int i = j + 1 + h; i < m; i += 2) x[i] -= M[i * kDaLd + j] * v;
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    for (int j = m - 1; j >= 0; j--) {
+        if (mine && h == 0) x[j] = x[j] / M[j * kDaLd + j];
+        __builtin_amdgcn_wave_barrier();
+        if (mine) {
+            const double v = x[j];
+            for (int i = h; i < j; i += 2) x[i] -= M[i * kDaLd + j] * v;
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
     return ok;
 }
 
@@ -156,12 +164,12 @@ __global__ __launch_bounds__(64) void da_radial_kernel(DaArgs a) {
     const long q = blockIdx.x;  // (V,K) line
     if (q >= a.VK) return;
     const int n = a.nR;
-    double* M0 = sh;                 // A^T, then its LU; later S^T, its LU, K H Pf
-    double* M1 = sh + kDaMat;        // F
+    double* M0 = sh;                 // A^T and its LU
+    double* M1 = sh + kDaMat;        // F; later S^T and its LU, then K H Pf
     double* M2 = sh + 2 * kDaMat;    // Pa, later Pf H^T and K
     double* M3 = sh + 3 * kDaMat;    // F Pa, later H Pf
-    double* M4 = sh + 4 * kDaMat;    // Pf
-    double* x = sh + 5 * kDaMat;     // the line's PSD [32]
+    double* M4 = M0;                 // Pf (A's factors are done with by then)
+    double* x = sh + 4 * kDaMat;     // the line's PSD [32]
     double* ob = x + kDaN;           // observations at the selected points
     int* idx = (int*)(ob + kDaN);    // selected points
     int* piv = idx + kDaN;
@@ -170,7 +178,7 @@ __global__ __launch_bounds__(64) void da_radial_kernel(DaArgs a) {
     double* Pa = a.Pa + (size_t)q * n * n;
 
     // zero the work matrices (padding included)
-    for (int t = lane; t < 5 * kDaMat; t += 64) sh[t] = 0.0;
+    for (int t = lane; t < 4 * kDaMat; t += 64) sh[t] = 0.0;
     __builtin_amdgcn_wave_barrier();
 
     // 1. the 1-D implicit diffusion matrices (getCalculationMatrix1DDiffusion,
@@ -280,10 +288,10 @@ __global__ __launch_bounds__(64) void da_radial_kernel(DaArgs a) {
     __builtin_amdgcn_wave_barrier();
 
     if (m > 0) {
-        // 3. S = H Pf H^T + R, stored transposed in M0; Pf H^T (n x m) in M2
+        // 3. S = H Pf H^T + R, stored transposed in M1; Pf H^T (n x m) in M2
         for (int t = lane; t < m * m; t += 64) {
             const int i = t / m, j = t % m;
-            M0[j * kDaLd + i] = M4[idx[i] * kDaLd + idx[j]] + (i == j ? a.observation_error : 0.0);
+            M1[j * kDaLd + i] = M4[idx[i] * kDaLd + idx[j]] + (i == j ? a.observation_error : 0.0);
         }
         for (int t = lane; t < n * m; t += 64) {
             const int i = t / m, r = t % m;
@@ -291,7 +299,7 @@ __global__ __launch_bounds__(64) void da_radial_kernel(DaArgs a) {
         }
         __builtin_amdgcn_wave_barrier();
         // 4. K = Pf H^T S^-1: row i of K solves S^T z = (row i of Pf H^T)^T
-        if (!lu_solve_rows(M0, M2, m, n, lane, piv)) {
+        if (!lu_solve_rows(M1, M2, m, n, lane, piv)) {
             if (lane == 0) raise_error(a.status, VERB_ERR_SINGULAR);
             return;
         }
@@ -310,10 +318,10 @@ __global__ __launch_bounds__(64) void da_radial_kernel(DaArgs a) {
             M3[r * kDaLd + j] = M4[idx[r] * kDaLd + j];
         }
         __builtin_amdgcn_wave_barrier();
-        gemm_lds(M0, M2, M3, n, n, m, false, lane);
+        gemm_lds(M1, M2, M3, n, n, m, false, lane);
         for (int t = lane; t < n * n; t += 64) {
             const int i = t / n, j = t % n;
-            Pa[t] = M4[i * kDaLd + j] - M0[i * kDaLd + j];
+            Pa[t] = M4[i * kDaLd + j] - M1[i * kDaLd + j];
         }
     } else {
         for (int t = lane; t < n * n; t += 64) Pa[t] = M4[(t / n) * kDaLd + t % n];
@@ -322,7 +330,9 @@ __global__ __launch_bounds__(64) void da_radial_kernel(DaArgs a) {
 
 }  // namespace
 
-size_t da_lds_bytes() { return (size_t)5 * kDaMat * sizeof(double) + 2 * kDaN * sizeof(double) + (2 * kDaN + 1) * sizeof(int); }
+size_t da_lds_bytes() {
+    return (size_t)4 * kDaMat * sizeof(double) + 2 * kDaN * sizeof(double) + (2 * kDaN + 1) * sizeof(int);
+}
 
 int launch_da_radial(const verb_grid& g, double* psd, const double* obs, double* Pa, double* F_out, int iP, double dt,
                      double model_error, double observation_error, int* status, hipStream_t stream) {
