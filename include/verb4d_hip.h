@@ -500,6 +500,42 @@ typedef struct {
 int verb_da_radial(verb_ctx* ctx, const verb_grid* g, double* psd, const double* obs, double* covariance,
                    double* model_matrix, const verb_da_params* params);
 
+/* Data assimilation in the (MLT, R) plane: the Kalman filter that
+ * DataAssimilationManagerConvection::assimilate runs on every (V,K) plane
+ * (src/DataAssimilationConvection.cpp:42-238), state = the plane's N = nP*nR
+ * points. Per plane: the model matrix M (the one-step upwind transport
+ * matrix of src/DataAssimilationHelper.cpp:25-102 with loss rate
+ * loss/psd, zero where psd >= saturation, for the Courant-split sub-step,
+ * raised to the sub-step count) is rebuilt into `model` when
+ * recompute_model is set and read from it otherwise; M1d = M^ceil(3/24/dt);
+ * Q = M1d diag(tmp) M1d^T + diag(model_error)/10 (tmp = model_error, (1 -
+ * c^2) model_error at the R ends, c = exp(-dt/correlation_time), those Q
+ * entries reset unless growing_Q_at_boundary); Pf = M Pa M^T + Q; S = H Pf
+ * H^T + R (R = observation_error I with use_log, else diag(obs *
+ * observation_error)); K = Pf H^T S^-1; x += K (obs - H x), in log space
+ * with use_log; Pa = Pf - K H Pf. psd, obs, loss, saturation: device
+ * [P][R][V][K] (saturation nullable: none saturated); covariance and model:
+ * device [V][K][N][N] (covariance zero before the first call: it starts at
+ * 5 model_error I). Planes plane_begin .. +plane_count (0: to the last).
+ * The dense products run on rocBLAS, the solves on rocSOLVER, in chunks of
+ * planes. VERB_ERR_DOMAIN (an unstable model matrix, a NaN analysis: the
+ * reference throws) is reported by verb_check_status; VERB_ERR_SINGULAR
+ * (a zero pivot in S, as for a plane without observations outside log
+ * space) is returned. */
+typedef struct {
+    double dt;
+    double correlation_time;
+    double model_error;
+    double observation_error;
+    int use_log;
+    int growing_Q_at_boundary;
+    int recompute_model;
+    long long plane_begin;
+    long long plane_count;
+} verb_da_conv_params;
+int verb_da_convection(verb_ctx* ctx, const verb_grid* g, double* psd, const double* obs, const double* loss,
+                       const double* saturation, double* covariance, double* model, const verb_da_conv_params* params);
+
 /* ------------------------------------------------------------------------
  * On-disk formats (host only; no device work). ndim is 3 (boundary arrays)
  * or 4 ([P][R][V][K] fields); sizes[0..ndim) in the array's index order,

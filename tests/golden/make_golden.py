@@ -306,11 +306,66 @@ def da_cases(seed):
     return flat
 
 
+def da_conv_cases(seed):
+    """The plane (MLT, R) filter by the reference's own model matrix and
+    matrix operations (oracle/ref_da.cpp: ref_da_conv_plane): small planes
+    with sub-step counts 1 and 2, 3-hour powers 1, 3, 4, 13 and 21, log space
+    on and off, the boundary Q growing or reset; each plane updated twice
+    (the second time from the first's covariance and model matrix)."""
+    import ctypes
+    rng = np.random.default_rng(seed)
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int)
+    L = O.ref_lib()
+    L.ref_da_conv_plane.argtypes = [ctypes.c_int, ctypes.c_int] + [dp] * 5 + [ip, dp, dp, dp, dp, ctypes.c_int] + [
+        ctypes.c_double] * 4 + [ctypes.c_int, ctypes.c_int, dp]
+    p = lambda a: a.ctypes.data_as(dp)  # noqa: E731
+    cases = []
+    # (Courant numbers up to 2*cfl; with two_steps one point at 1.5 makes
+    # the sub-step count 2, so every point's sub-step number is <= cfl)
+    for nP, nR, dt, use_log, growing, cfl, two_steps in ((6, 5, 1.0, 0, 0, 0.45, 0), (6, 5, 0.05, 0, 1, 0.45, 0),
+                                                         (6, 5, 0.05, 1, 0, 0.7, 1), (8, 6, 0.01, 0, 0, 0.7, 1),
+                                                         (6, 5, 0.006, 1, 1, 0.45, 0), (24, 10, 0.04, 0, 0, 0.7, 1)):
+        P = np.ascontiguousarray(np.broadcast_to(np.linspace(0, 2 * np.pi, nP)[:, None], (nP, nR)))
+        R = np.ascontiguousarray(np.broadcast_to(np.linspace(1.5, 6.6, nR)[None, :], (nP, nR)))
+        dP, dR = P[1, 0] - P[0, 0], R[0, 1] - R[0, 0]
+        VP = rng.uniform(-1, 1, (nP, nR)) * dP / dt * cfl
+        VR = rng.uniform(-1, 1, (nP, nR)) * dR / dt * cfl
+        if two_steps:
+            VP[2, 2], VR[2, 2] = -0.9 * dP / dt, 0.6 * dR / dt
+        loss = -rng.uniform(0, 0.01, (nP, nR)) / dt
+        sat = (rng.random((nP, nR)) < 0.1).astype(np.int32)
+        x0 = np.exp(-(R - 4.5) ** 2 / 0.5) * (1 + 0.1 * rng.random((nP, nR)))
+        obs = np.where(rng.random((nP, nR)) < 0.3, x0 * (1 + 0.2 * rng.standard_normal((nP, nR))), np.nan)
+        N = nP * nR
+        x = x0.copy()
+        Pa = np.zeros((N, N))
+        M = np.zeros((N, N))
+        out = dict(nP=nP, nR=nR, P=P, R=R, VP=VP, VR=VR, loss=loss, sat=sat, x0=x0, obs=obs, dt=dt, corr=3.0,
+                   me=1e-2, oe=4e-2, use_log=use_log, growing=growing)
+        for rep in range(2):
+            Pf = np.zeros((N, N))
+            st = L.ref_da_conv_plane(nP, nR, p(P), p(R), p(VP), p(VR), p(loss), sat.ctypes.data_as(ip), p(obs), p(x),
+                                     p(Pa), p(M), rep, dt, 3.0, 1e-2, 4e-2, use_log, growing, p(Pf))
+            assert st == 0, st
+            out["x%d" % (rep + 1)] = x.copy()
+            out["P%d" % (rep + 1)] = Pa.copy()
+            out["Pf%d" % (rep + 1)] = Pf
+        out["M"] = M.copy()
+        cases.append(out)
+    flat = {"count": len(cases)}
+    for i, c in enumerate(cases):
+        for k, v in c.items():
+            flat["%d_%s" % (i, k)] = v
+    return flat
+
+
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "da":
         if not O.ref_available():
             O.build()
         save("da_lines.npz", da_cases(31))
+        save("da_planes.npz", da_conv_cases(41))
         return
     if len(sys.argv) > 1 and sys.argv[1] == "options":
         option_fixtures()

@@ -27,6 +27,7 @@ SOURCES = [
     ("driver.hip", ["-ffp-contract=off"]),
     ("remap.hip", ["-ffp-contract=off"]),
     ("da.hip", []),
+    ("da_conv.hip", []),
     ("api.hip", []),
     ("noalloc.hip", []),
     ("unet.hip", []),
@@ -94,7 +95,8 @@ def build(verbose=False, incremental=False, jobs=8):
     if failed:
         raise RuntimeError("hipcc failed for %s" % ", ".join(failed))
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs +
+         ["-L/opt/rocm/lib", "-lrocblas", "-lrocsolver", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIB)
     return LIB
 

@@ -54,6 +54,8 @@ struct verb_ctx {
     unsigned long long graph_gen = 0;  // scratch_gen at capture
     // the uneven extents last uploaded (first column, unknowns, dispatch order)
     std::vector<int> uneven_k0;
+    // rocBLAS handle of the plane data assimilation (created on first use)
+    void* blas = nullptr;
 };
 
 namespace {
@@ -179,6 +181,7 @@ void mark_end(verb_ctx* c, int idx) {
 
 namespace verb {
 int ctx_error(verb_ctx* c, int code, const char* what) { return set_err(c, code, what); }
+void* ctx_scratch(verb_ctx* c, int slot, size_t bytes) { return scratch(c, slot, bytes); }
 }  // namespace verb
 
 extern "C" {
@@ -214,6 +217,7 @@ int verb_ctx_destroy(verb_ctx* c) {
     hipStreamSynchronize(c->stream);
     if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
     if (c->graph) hipGraphDestroy(c->graph);
+    verb::da_conv_release(c->blas);
     for (auto& b : c->bufs)
         if (b.first) hipFree(b.first);
     hipFree(c->status);
@@ -775,6 +779,7 @@ int verb_step_graph(verb_ctx* c, const verb_grid* g, double* psd, const int* k0)
     if (rc) return rc;
     if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
     if (c->graph) hipGraphDestroy(c->graph);
+    verb::da_conv_release(c->blas);
     c->graph_exec = nullptr;
     c->graph = nullptr;
     const bool saved_timing = c->timing;
@@ -864,6 +869,17 @@ int verb_da_radial(verb_ctx* c, const verb_grid* g, double* psd, const double* o
     int rc = verb::launch_da_radial(*g, psd, obs, covariance, model_matrix, p->p_index, p->dt, p->model_error,
                                     p->observation_error, c->status, c->stream);
     return rc ? set_err(c, rc, "data assimilation launch rejected (3 <= nR <= 32)") : VERB_OK;
+}
+
+int verb_da_convection(verb_ctx* c, const verb_grid* g, double* psd, const double* obs, const double* loss,
+                       const double* saturation, double* covariance, double* model, const verb_da_conv_params* p) {
+    if (!c || !grid_ok(g) || !psd || !obs || !loss || !covariance || !model || !p || !g->Pg || !g->Rg || !g->VP ||
+        !g->VR)
+        return VERB_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = verb::run_da_convection(c, &c->blas, c->stream, c->status, *g, psd, obs, loss, saturation, covariance,
+                                     model, *p);
+    return rc == VERB_ERR_ARG ? set_err(c, rc, "data assimilation arguments rejected") : rc;
 }
 
 int verb_cubic1d(verb_ctx* c, int n, const double* x_old, double* values, const double* x_new) {

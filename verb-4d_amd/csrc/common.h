@@ -155,6 +155,14 @@ int launch_stability_min(const verb_grid& g, double* part, double* out, hipStrea
 int launch_da_radial(const verb_grid& g, double* psd, const double* obs, double* Pa, double* F_out, int iP, double dt,
                      double model_error, double observation_error, int* status, hipStream_t stream);
 
+// da_conv.hip: the (MLT, R) plane Kalman filter on rocBLAS / rocSOLVER
+int run_da_convection(verb_ctx* c, void** blas_slot, hipStream_t st, int* status, const verb_grid& g, double* psd,
+                      const double* obs, const double* loss, const double* satd, double* cov, double* model,
+                      const verb_da_conv_params& p);
+void da_conv_release(void* blas);
+// a context's grow-only scratch slot (api.hip)
+void* ctx_scratch(verb_ctx* c, int slot, size_t bytes);
+
 // remap.hip: Cubic1D on every (P,V,K) line (nR <= 105)
 int launch_remap(double* psd, const double* lold, const double* lnew, int nP, int nR, int nV, int nK, int* status,
                  hipStream_t stream);

@@ -115,6 +115,13 @@ class DaParams(ctypes.Structure):
                 ("p_index", ctypes.c_int)]
 
 
+class DaConvParams(ctypes.Structure):
+    _fields_ = [("dt", ctypes.c_double), ("correlation_time", ctypes.c_double), ("model_error", ctypes.c_double),
+                ("observation_error", ctypes.c_double), ("use_log", ctypes.c_int),
+                ("growing_Q_at_boundary", ctypes.c_int), ("recompute_model", ctypes.c_int),
+                ("plane_begin", ctypes.c_longlong), ("plane_count", ctypes.c_longlong)]
+
+
 class Tuning(ctypes.Structure):
     """ctypes mirror of ``verb_tuning``."""
 
@@ -215,6 +222,8 @@ _SIGNATURES = [
     ("verb_cubic1d", ctypes.c_int, [_P, ctypes.c_int, _P, _P, _P]),
     ("verb_da_radial", ctypes.c_int,
      [_P, ctypes.POINTER(VerbGrid), _P, _P, _P, _P, ctypes.POINTER(DaParams)]),
+    ("verb_da_convection", ctypes.c_int,
+     [_P, ctypes.POINTER(VerbGrid), _P, _P, _P, _P, _P, _P, ctypes.POINTER(DaConvParams)]),
     ("verb_plt_write", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, _P, _P, ctypes.c_char_p, ctypes.c_char_p]),
     ("verb_plt_write_grid", ctypes.c_int,
      [ctypes.c_char_p, ctypes.c_int, _P, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_char_p), _P,
