@@ -517,8 +517,12 @@ int verb_da_radial(verb_ctx* ctx, const verb_grid* g, double* psd, const double*
  * [P][R][V][K] (saturation nullable: none saturated); covariance and model:
  * device [V][K][N][N] (covariance zero before the first call: it starts at
  * 5 model_error I). Planes plane_begin .. +plane_count (0: to the last).
- * The dense products run on rocBLAS, the solves on rocSOLVER, in chunks of
- * planes. VERB_ERR_DOMAIN (an unstable model matrix, a NaN analysis: the
+ * process_noise (device [V][K][N][N], nullable): when given, Q is built into
+ * it with the model matrix (recompute_model) and read from it otherwise, as
+ * it depends only on M, dt and the parameters (the reference rebuilds it on
+ * every call: the same values).
+ * The dense products run on rocBLAS, S's LU and inverse on rocSOLVER, in
+ * chunks of planes. VERB_ERR_DOMAIN (an unstable model matrix, a NaN analysis: the
  * reference throws) is reported by verb_check_status; VERB_ERR_SINGULAR
  * (a zero pivot in S, as for a plane without observations outside log
  * space) is returned. */
@@ -532,6 +536,7 @@ typedef struct {
     int recompute_model;
     long long plane_begin;
     long long plane_count;
+    double* process_noise;
 } verb_da_conv_params;
 int verb_da_convection(verb_ctx* ctx, const verb_grid* g, double* psd, const double* obs, const double* loss,
                        const double* saturation, double* covariance, double* model, const verb_da_conv_params* params);

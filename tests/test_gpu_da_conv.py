@@ -7,7 +7,8 @@ a loss array and a saturation density, observations at some points of
 every plane and at all points of one; the model matrices rebuilt on the
 first call and reused on the second (which starts from the first's
 covariance), in linear and in log space, with the boundary Q reset and
-growing. A plane without observations outside log space makes the
+growing, with Q kept between the calls (linear space) or rebuilt (log
+space). A plane without observations outside log space makes the
 reference's solve exit; here the call returns VERB_ERR_SINGULAR.
 
 Bars (the products' summation order differs from OpenBLAS's): the model
@@ -112,12 +113,15 @@ def test_da_convection_vs_oracle(ctx, use_log, growing):
     tobs, tloss, tsat = dev(obs), dev(loss), dev(satd)
     cov = torch.zeros((NQ, N, N), dtype=torch.float64, device="cuda")
     model = torch.zeros((NQ, N, N), dtype=torch.float64, device="cuda")
+    # the linear-space case keeps Q between the calls, the log-space one rebuilds it
+    qn = None if use_log else torch.zeros((NQ, N, N), dtype=torch.float64, device="cuda")
     x = w.psd0.copy()
     Pa = np.zeros((NQ, N, N))
     M = np.zeros((NQ, N, N))
     for rep in range(2):
         dg.da_convection(tobs, tloss, cov, model, dt, corr, me, oe, saturation=tsat, use_log=use_log,
-                         growing_Q_at_boundary=growing, recompute_model=(rep == 0), plane_begin=q0, plane_count=nq)
+                         growing_Q_at_boundary=growing, recompute_model=(rep == 0), plane_begin=q0, plane_count=nq,
+                         process_noise=qn)
         dg.check_status()
         x, Pa, M = _oracle_planes(w, x, loss, satd, obs, Pa, M, planes, dt, corr, me, oe, use_log, growing,
                                   rep == 0)
@@ -134,6 +138,8 @@ def test_da_convection_vs_oracle(ctx, use_log, growing):
         keep = np.r_[0:q0, q0 + nq:NQ]
         assert np.array_equal(flat[:, :, keep], w.psd0.reshape(w.nP, w.nR, NQ)[:, :, keep])
         assert not np.any(gc[keep]) and not np.any(gm[keep])
+        if qn is not None:
+            assert not np.any(qn.cpu().numpy()[keep]) and np.all(qn.cpu().numpy()[q0].diagonal() > 0)
     # the plane observed everywhere moved towards its observations
     q = q0
     before = np.abs(_plane(w.psd0, q, w.nK) - _plane(obs, q, w.nK)).sum()

@@ -160,6 +160,8 @@ __global__ __launch_bounds__(kDcThreads) void dc_q_diag(double* Q, int nP, int n
 }
 
 // the analysis covariance starts at 5 modelError I where it is all zero
+// (a covariance that was ever set has a nonzero diagonal: that is checked
+// first, the whole matrix only when the diagonal is zero)
 __global__ __launch_bounds__(kDcThreads) void dc_pa_init(double* Pa, int N, double model_error) {
     const int c = blockIdx.x;
     double* P = Pa + (size_t)c * N * N;
@@ -167,6 +169,10 @@ __global__ __launch_bounds__(kDcThreads) void dc_pa_init(double* Pa, int N, doub
     if (threadIdx.x == 0) nz = 0;
     __syncthreads();
     int mine = 0;
+    for (int i = threadIdx.x; i < N; i += kDcThreads) mine |= (P[(size_t)i * N + i] != 0.0);
+    if (mine) atomicOr(&nz, 1);
+    __syncthreads();
+    if (nz) return;
     for (size_t t = threadIdx.x; t < (size_t)N * N; t += kDcThreads) mine |= (P[t] != 0.0);
     if (mine) atomicOr(&nz, 1);
     __syncthreads();
@@ -174,39 +180,53 @@ __global__ __launch_bounds__(kDcThreads) void dc_pa_init(double* Pa, int N, doub
         for (int i = threadIdx.x; i < N; i += kDcThreads) P[(size_t)i * N + i] = 5 * model_error;
 }
 
-// observations of each plane: the finite ones in (P,R) order, their count,
-// the forecast and the observed values (log with use_log)
+// observations of each plane: the finite ones in (P,R) order (a block-wide
+// compaction by wave ballots), their count, the forecast, the observed
+// values and the innovation data - H x (log space with use_log)
 __global__ __launch_bounds__(kDcThreads) void dc_observe(PlaneArgs a, const double* obs, int use_log, int* idx,
-                                                         int* m, double* x, double* data) {
+                                                         int* m, double* x, double* data, double* innov) {
     const int c = blockIdx.x;
     const long q = a.q0 + c;
     const int N = a.nP * a.nR;
-    __shared__ int cnt;
-    if (threadIdx.x == 0) {
-        int k = 0;
-        for (int t = 0; t < N; t++) {
-            const double o = obs[at(a, t / a.nR, t % a.nR, q)];
-            if (isfinite(o)) {
-                idx[(size_t)c * N + k] = t;
-                data[(size_t)c * N + k] = use_log ? log(o) : o;
-                k++;
-            }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __shared__ int wcount[kDcThreads / 64];
+    int base = 0;
+    for (int t0 = 0; t0 < N; t0 += kDcThreads) {
+        const int t = t0 + threadIdx.x;
+        double o = 0.0, v = 0.0;
+        bool f = false;
+        if (t < N) {
+            const size_t g = at(a, t / a.nR, t % a.nR, q);
+            o = obs[g];
+            v = a.psd[g];
+            f = isfinite(o);
+            x[(size_t)c * N + t] = use_log ? log(v) : v;
         }
-        cnt = k;
-        m[c] = k;
+        const unsigned long long bal = __ballot(f);
+        if (lane == 0) wcount[wave] = __popcll(bal);
+        __syncthreads();
+        int off = base;
+        for (int w = 0; w < wave; w++) off += wcount[w];
+        int total = 0;
+        for (int w = 0; w < kDcThreads / 64; w++) total += wcount[w];
+        if (f) {
+            const int k = off + __popcll(bal & ((1ull << lane) - 1));
+            const double d = use_log ? log(o) : o;
+            idx[(size_t)c * N + k] = t;
+            data[(size_t)c * N + k] = d;
+            innov[(size_t)c * N + k] = d - (use_log ? log(v) : v);
+        }
+        base += total;
+        __syncthreads();
     }
-    __syncthreads();
-    for (int t = threadIdx.x; t < N; t += kDcThreads) {
-        const double v = a.psd[at(a, t / a.nR, t % a.nR, q)];
-        x[(size_t)c * N + t] = use_log ? log(v) : v;
-    }
-    (void)cnt;
+    if (threadIdx.x == 0) m[c] = base;
 }
 
 // S (mm x mm, padded with the identity), Pf H^T (N x mm, zero padded) and
 // H Pf (mm x N, zero padded); R = diag(observationError) with use_log, else
 // diag(obs * observationError). A plane without observations gets the
-// reference's single all-zero pseudo-observation of value 0.
+// reference's single all-zero pseudo-observation of value 0. Grid: planes x
+// slices of the three outputs.
 __global__ __launch_bounds__(kDcThreads) void dc_select(const double* Pf, const int* idx, const int* m,
                                                         const double* data, int N, int mm, int use_log,
                                                         double observation_error, double* S, double* PH,
@@ -219,7 +239,8 @@ __global__ __launch_bounds__(kDcThreads) void dc_select(const double* Pf, const 
     double* Sc = S + (size_t)c * mm * mm;
     double* PHc = PH + (size_t)c * N * mm;
     double* HPc = HP + (size_t)c * mm * N;
-    for (int t = threadIdx.x; t < mm * mm; t += kDcThreads) {
+    const int s0 = blockIdx.y * kDcThreads + threadIdx.x, ss = gridDim.y * kDcThreads;
+    for (int t = s0; t < mm * mm; t += ss) {
         const int i = t / mm, j = t % mm;
         double v;
         if (i < mc && j < mc) {
@@ -233,37 +254,48 @@ __global__ __launch_bounds__(kDcThreads) void dc_select(const double* Pf, const 
         }
         Sc[(size_t)i * mm + j] = v;
     }
-    for (int t = threadIdx.x; t < N * mm; t += kDcThreads) {
+    for (int t = s0; t < N * mm; t += ss) {
         const int i = t / mm, r = t % mm;
         PHc[t] = r < mc ? P[(size_t)i * N + id[r]] : 0.0;
     }
-    for (int t = threadIdx.x; t < mm * N; t += kDcThreads) {
+    for (int t = s0; t < mm * N; t += ss) {
         const int r = t / N, j = t % N;
         HPc[t] = r < mc ? P[(size_t)id[r] * N + j] : 0.0;
     }
 }
 
-// x += K (data - H x); NaN -> error (the reference throws); back from log
+// the identity in each plane's mm x mm block (the right-hand sides that
+// give S^-1)
+__global__ __launch_bounds__(kDcThreads) void dc_identity(double* I, int mm) {
+    double* Ic = I + (size_t)blockIdx.x * mm * mm;
+    for (int t = blockIdx.y * kDcThreads + threadIdx.x; t < mm * mm; t += gridDim.y * kDcThreads)
+        Ic[t] = (t / mm == t % mm) ? 1.0 : 0.0;
+}
+
+// x += K (data - H x), a wavefront per row of K (row-major N x mm, plane
+// stride kstride); NaN -> error (the reference throws); back from log
 // space; the plane written back into the PSD
 __global__ __launch_bounds__(kDcThreads) void dc_analysis(PlaneArgs a, double* psd, const double* x, const double* K,
-                                                          const int* idx, const int* m, const double* data, int mm,
+                                                          size_t kstride, const int* m, const double* innov, int mm,
                                                           int use_log, int* status) {
     const int c = blockIdx.x;
     const long q = a.q0 + c;
     const int N = a.nP * a.nR;
     const int mc = m[c];
-    const double* xc = x + (size_t)c * N;
-    const double* Kc = K + (size_t)c * N * mm;
-    const int* id = idx + (size_t)c * N;
-    const double* dv = data + (size_t)c * N;
-    for (int i = threadIdx.x; i < N; i += kDcThreads) {
+    const int lane = threadIdx.x & 63;
+    const double* Kc = K + (size_t)c * kstride;
+    const double* in = innov + (size_t)c * N;
+    for (int i = blockIdx.y * (kDcThreads / 64) + (threadIdx.x >> 6); i < N; i += gridDim.y * (kDcThreads / 64)) {
         double corr = 0.0;
-        for (int t = 0; t < mc; t++) corr = fma(Kc[(size_t)i * mm + t], dv[t] - xc[id[t]], corr);
-        if (mc == 0 && use_log) corr = 0.0 * (0.0 - log(0.0));  // log(0) - 0 times a zero gain: NaN
-        double v = xc[i] + corr;
-        if (isnan(v)) raise_error(status, VERB_ERR_DOMAIN);
-        if (use_log) v = pow(exp(1.0), v);
-        psd[at(a, i / a.nR, i % a.nR, q)] = v;
+        for (int t = lane; t < mc; t += 64) corr = fma(Kc[(size_t)i * mm + t], in[t], corr);
+        for (int o = 32; o > 0; o >>= 1) corr += __shfl_xor(corr, o);
+        if (lane == 0) {
+            if (mc == 0 && use_log) corr = 0.0 * (0.0 - log(0.0));  // log(0) - 0 times a zero gain: NaN
+            double v = x[(size_t)c * N + i] + corr;
+            if (isnan(v)) raise_error(status, VERB_ERR_DOMAIN);
+            if (use_log) v = pow(exp(1.0), v);
+            psd[at(a, i / a.nR, i % a.nR, q)] = v;
+        }
     }
 }
 
@@ -273,16 +305,21 @@ struct Blas {
     rocblas_handle h = nullptr;
 };
 
-// row-major C (m x n) = alpha A (m x k) op(B) + beta C, op(B) = B (k x n)
-// or B^T (B stored n x k), strided over `batch` matrices
+// row-major D (m x n) = alpha A (m x k) op(B) + beta C, op(B) = B (k x n)
+// or B^T (B stored n x k), strided over `batch` matrices; D may be C
+rocblas_status gemm_rm(rocblas_handle h, bool bT, int m, int n, int k, double alpha, const double* A, long sA,
+                       const double* B, long sB, double beta, const double* C, long sC, double* D, long sD,
+                       int batch) {
+    // column-major: D^T (n x m) = op(B)^T A^T + beta C^T
+    const rocblas_datatype f = rocblas_datatype_f64_r;
+    return rocblas_gemm_strided_batched_ex(h, bT ? rocblas_operation_transpose : rocblas_operation_none,
+                                           rocblas_operation_none, n, m, k, &alpha, B, f, bT ? k : n, sB, A, f, k, sA,
+                                           &beta, C, f, n, sC, D, f, n, sD, batch, f, rocblas_gemm_algo_standard, 0,
+                                           0);
+}
 rocblas_status gemm_rm(rocblas_handle h, bool bT, int m, int n, int k, double alpha, const double* A, long sA,
                        const double* B, long sB, double beta, double* C, long sC, int batch) {
-    // column-major: C^T (n x m) = op(B)^T A^T
-    if (!bT)
-        return rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, n, m, k, &alpha, B, n,
-                                             sB, A, k, sA, &beta, C, n, sC, batch);
-    return rocblas_dgemm_strided_batched(h, rocblas_operation_transpose, rocblas_operation_none, n, m, k, &alpha, B, k,
-                                         sB, A, k, sA, &beta, C, n, sC, batch);
+    return gemm_rm(h, bT, m, n, k, alpha, A, sA, B, sB, beta, C, sC, C, sC, batch);
 }
 
 // res = A^p for `batch` planes (stride N*N); w0, w1: squares (ping-pong),
@@ -369,7 +406,9 @@ int run_da_convection(verb_ctx* c, void** blas_slot, hipStream_t st, int* status
     int* d_idx = (int*)ctx_scratch(c, 116, sizeof(int) * N * chunk);
     double* d_x = (double*)ctx_scratch(c, 117, sizeof(double) * N * chunk);
     double* d_data = (double*)ctx_scratch(c, 118, sizeof(double) * N * chunk);
-    if (!d_nt || !d_m || !d_idx || !d_x || !d_data) return ctx_error(c, VERB_ERR_NOMEM, "device allocation failed");
+    double* d_innov = (double*)ctx_scratch(c, 125, sizeof(double) * N * chunk);
+    if (!d_nt || !d_m || !d_idx || !d_x || !d_data || !d_innov)
+        return ctx_error(c, VERB_ERR_NOMEM, "device allocation failed");
     const double cf = exp(-p.dt / p.correlation_time);
     const int p1 = (int)std::ceil(3. / 24 / p.dt);
     std::vector<int> nt(chunk), m(chunk);
@@ -396,26 +435,34 @@ int run_da_convection(verb_ctx* c, void** blas_slot, hipStream_t st, int* status
                 i0 = i1;
             }
         }
-        // M1d = M^p1 into W[3] (or M itself), Q = M1d diag(tmp) M1d^T + diag(D)/10 into W[1]
-        const double* M1d = Mq;
-        if (p1 != 1) {
-            if (power_rm(b->h, st, Mq, p1, N, C, W[3], W[0], W[1], W[2]) != rocblas_status_success)
+        // M1d = M^p1 into W[3] (or M itself), Q = M1d diag(tmp) M1d^T +
+        // diag(D)/10 into W[1] (through the caller's Q array when given:
+        // built with the model matrix, read otherwise)
+        double* Qc = p.process_noise ? p.process_noise + (size_t)q0 * NN : nullptr;
+        if (!Qc || p.recompute_model) {
+            double* Qdst = Qc ? Qc : W[1];
+            const double* M1d = Mq;
+            if (p1 != 1) {
+                if (power_rm(b->h, st, Mq, p1, N, C, W[3], W[0], W[1], W[2]) != rocblas_status_success)
+                    return ctx_error(c, VERB_ERR_HIP, "rocBLAS dgemm failed");
+                M1d = W[3];
+            }
+            hipLaunchKernelGGL(dc_scale_cols, dim3(grid_blocks(NN * C)), dim3(kDcThreads), 0, st, M1d, W[0], nP, nR,
+                               p.model_error, cf, NN * C);
+            if (gemm_rm(b->h, true, N, N, N, 1.0, W[0], NN, M1d, NN, 0.0, Qdst, NN, C) != rocblas_status_success)
                 return ctx_error(c, VERB_ERR_HIP, "rocBLAS dgemm failed");
-            M1d = W[3];
+            hipLaunchKernelGGL(dc_q_diag, dim3(grid_blocks(NN * C)), dim3(kDcThreads), 0, st, Qdst, nP, nR,
+                               p.model_error, cf, p.growing_Q_at_boundary, NN * C);
         }
-        hipLaunchKernelGGL(dc_scale_cols, dim3(grid_blocks(NN * C)), dim3(kDcThreads), 0, st, M1d, W[0], nP, nR,
-                           p.model_error, cf, NN * C);
-        if (gemm_rm(b->h, true, N, N, N, 1.0, W[0], NN, M1d, NN, 0.0, W[1], NN, C) != rocblas_status_success)
-            return ctx_error(c, VERB_ERR_HIP, "rocBLAS dgemm failed");
-        hipLaunchKernelGGL(dc_q_diag, dim3(grid_blocks(NN * C)), dim3(kDcThreads), 0, st, W[1], nP, nR, p.model_error,
-                           cf, p.growing_Q_at_boundary, NN * C);
-        // Pf = (M Pa) M^T + Q into W[1]
+        // Pf = (M Pa) M^T + Q into W[1] (Q read from the caller's array or from W[1])
         hipLaunchKernelGGL(dc_pa_init, dim3(C), dim3(kDcThreads), 0, st, Pa, N, p.model_error);
         if (gemm_rm(b->h, false, N, N, N, 1.0, Mq, NN, Pa, NN, 0.0, W[2], NN, C) != rocblas_status_success ||
-            gemm_rm(b->h, true, N, N, N, 1.0, W[2], NN, Mq, NN, 1.0, W[1], NN, C) != rocblas_status_success)
+            gemm_rm(b->h, true, N, N, N, 1.0, W[2], NN, Mq, NN, 1.0, Qc ? Qc : W[1], NN, W[1], NN, C) !=
+                rocblas_status_success)
             return ctx_error(c, VERB_ERR_HIP, "rocBLAS dgemm failed");
         // observations
-        hipLaunchKernelGGL(dc_observe, dim3(C), dim3(kDcThreads), 0, st, a, obs, p.use_log, d_idx, d_m, d_x, d_data);
+        hipLaunchKernelGGL(dc_observe, dim3(C), dim3(kDcThreads), 0, st, a, obs, p.use_log, d_idx, d_m, d_x, d_data,
+                           d_innov);
         if (hipMemcpyAsync(m.data(), d_m, sizeof(int) * C, hipMemcpyDeviceToHost, st) != hipSuccess ||
             hipStreamSynchronize(st) != hipSuccess)
             return ctx_error(c, VERB_ERR_HIP, "observation count read-back failed");
@@ -426,28 +473,37 @@ int run_da_convection(verb_ctx* c, void** blas_slot, hipStream_t st, int* status
         rocblas_int* piv = (rocblas_int*)ctx_scratch(c, 122, sizeof(rocblas_int) * (size_t)mm * C);
         rocblas_int* info = (rocblas_int*)ctx_scratch(c, 123, sizeof(rocblas_int) * C);
         if (!S || !PH || !HP || !piv || !info) return ctx_error(c, VERB_ERR_NOMEM, "device allocation failed");
-        hipLaunchKernelGGL(dc_select, dim3(C), dim3(kDcThreads), 0, st, W[1], d_idx, d_m, d_data, N, mm, p.use_log,
-                           p.observation_error, S, PH, HP);
-        // K = Pf H^T S^-1: S row-major is S^T column-major; Pf H^T row-major
-        // (N x mm) is (Pf H^T)^T column-major (mm x N): solving S^T Y = (Pf H^T)^T
-        // leaves K^T column-major = K row-major in PH
+        const int ysel = (int)std::min<size_t>(256, ((size_t)mm * mm + 2 * (size_t)N * mm + 4095) / 4096);
+        hipLaunchKernelGGL(dc_select, dim3(C, ysel), dim3(kDcThreads), 0, st, W[1], d_idx, d_m, d_data, N, mm,
+                           p.use_log, p.observation_error, S, PH, HP);
+        // K = Pf H^T S^-1 (trans_solve: dgesv on S^T with the rows of Pf H^T
+        // as right-hand sides). S row-major is S^T column-major: its LU with
+        // partial pivoting and the solve with the identity give S^-T
+        // column-major = S^-1 row-major; then K = (Pf H^T) S^-1 by one GEMM
+        // (row-major N x mm in W[0], plane stride N*N)
+        double* Si = (double*)ctx_scratch(c, 124, sizeof(double) * (size_t)mm * mm * C);
+        if (!Si) return ctx_error(c, VERB_ERR_NOMEM, "device allocation failed");
+        hipLaunchKernelGGL(dc_identity, dim3(C, std::min(64, (mm * mm + 4095) / 4096)), dim3(kDcThreads), 0, st, Si,
+                           mm);
         if (rocsolver_dgetrf_strided_batched(b->h, mm, mm, S, mm, (rocblas_stride)mm * mm, piv, mm, info, C) !=
-                rocblas_status_success ||
-            rocsolver_dgetrs_strided_batched(b->h, rocblas_operation_none, mm, N, S, mm, (rocblas_stride)mm * mm, piv,
-                                             mm, PH, mm, (rocblas_stride)N * mm, C) != rocblas_status_success)
-            return ctx_error(c, VERB_ERR_HIP, "rocSOLVER solve failed");
+            rocblas_status_success)
+            return ctx_error(c, VERB_ERR_HIP, "rocSOLVER getrf failed");
         std::vector<rocblas_int> hinfo(C);
         if (hipMemcpyAsync(hinfo.data(), info, sizeof(rocblas_int) * C, hipMemcpyDeviceToHost, st) != hipSuccess ||
             hipStreamSynchronize(st) != hipSuccess)
             return ctx_error(c, VERB_ERR_HIP, "solve status read-back failed");
         for (int i = 0; i < C; i++)
             if (hinfo[i] != 0) return ctx_error(c, VERB_ERR_SINGULAR, "trans_solve: zero pivot");
-        hipLaunchKernelGGL(dc_analysis, dim3(C), dim3(kDcThreads), 0, st, a, psd, d_x, PH, d_idx, d_m, d_data, mm,
-                           p.use_log, status);
+        if (rocsolver_dgetrs_strided_batched(b->h, rocblas_operation_none, mm, mm, S, mm, (rocblas_stride)mm * mm, piv,
+                                             mm, Si, mm, (rocblas_stride)mm * mm, C) != rocblas_status_success)
+            return ctx_error(c, VERB_ERR_HIP, "rocSOLVER getrs failed");
+        if (gemm_rm(b->h, false, N, mm, mm, 1.0, PH, (long)N * mm, Si, (long)mm * mm, 0.0, W[0], NN, C) !=
+            rocblas_status_success)
+            return ctx_error(c, VERB_ERR_HIP, "rocBLAS dgemm failed");
+        hipLaunchKernelGGL(dc_analysis, dim3(C, (N + 15) / 16), dim3(kDcThreads), 0, st, a, psd, d_x, W[0], NN, d_m,
+                           d_innov, mm, p.use_log, status);
         // Pa = Pf - K (H Pf)
-        if (hipMemcpyAsync(Pa, W[1], sizeof(double) * NN * C, hipMemcpyDeviceToDevice, st) != hipSuccess)
-            return ctx_error(c, VERB_ERR_HIP, "copy failed");
-        if (gemm_rm(b->h, false, N, N, mm, -1.0, PH, (long)N * mm, HP, (long)mm * N, 1.0, Pa, NN, C) !=
+        if (gemm_rm(b->h, false, N, N, mm, -1.0, W[0], NN, HP, (long)mm * N, 1.0, W[1], NN, Pa, NN, C) !=
             rocblas_status_success)
             return ctx_error(c, VERB_ERR_HIP, "rocBLAS dgemm failed");
     }

@@ -119,7 +119,8 @@ class DaConvParams(ctypes.Structure):
     _fields_ = [("dt", ctypes.c_double), ("correlation_time", ctypes.c_double), ("model_error", ctypes.c_double),
                 ("observation_error", ctypes.c_double), ("use_log", ctypes.c_int),
                 ("growing_Q_at_boundary", ctypes.c_int), ("recompute_model", ctypes.c_int),
-                ("plane_begin", ctypes.c_longlong), ("plane_count", ctypes.c_longlong)]
+                ("plane_begin", ctypes.c_longlong), ("plane_count", ctypes.c_longlong),
+                ("process_noise", ctypes.c_void_p)]
 
 
 class Tuning(ctypes.Structure):

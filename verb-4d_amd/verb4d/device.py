@@ -133,18 +133,20 @@ class DeviceGrid:
 
     def da_convection(self, obs, loss, covariance, model, dt, correlation_time, model_error, observation_error,
                       saturation=None, use_log=False, growing_Q_at_boundary=False, recompute_model=True,
-                      plane_begin=0, plane_count=0):
+                      plane_begin=0, plane_count=0, process_noise=None):
         """verb_da_convection: the Kalman filter of every (V,K) plane in
         plane_begin .. +plane_count (0: to the last); obs, loss, saturation
         [P][R][V][K] and covariance, model [V][K][N][N] device tensors
         (covariance zero before the first call; model written when
-        recompute_model, read otherwise). The solver passes its Sources
-        array as loss (src/VERB4D_Solver.cpp:1246)."""
+        recompute_model, read otherwise; process_noise, optional, the same
+        for Q). The solver passes its Sources array as loss
+        (src/VERB4D_Solver.cpp:1246)."""
         import torch
         from . import DaConvParams
         p = DaConvParams(float(dt), float(correlation_time), float(model_error), float(observation_error),
                          int(bool(use_log)), int(bool(growing_Q_at_boundary)), int(bool(recompute_model)),
-                         int(plane_begin), int(plane_count))
+                         int(plane_begin), int(plane_count),
+                         process_noise.data_ptr() if process_noise is not None else None)
         torch.cuda.current_stream().synchronize()
         self.ctx.check(lib().verb_da_convection(self.ctx.handle, ctypes.byref(self.grid), self.psd.data_ptr(),
                                                 obs.data_ptr(), loss.data_ptr(),
