@@ -148,7 +148,9 @@ LocalBuffers local_buffers(verb_ctx* c, const verb_grid* g, double* lu_cache, in
     lb.w.diag = (double*)scratch(c, base_slot + 5, 9 * n * nodes * sizeof(double));
     lb.w.y = (double*)scratch(c, base_slot + 6, n * nodes * sizeof(double));
     lb.w.uext = (int*)scratch(c, base_slot + 7, n * nodes * sizeof(int));
-    lb.w.queue = c->lu_persistent ? (int*)scratch(c, kSlotQueue, sizeof(int)) : nullptr;
+    // the persistent LU's counters: node slots, back-substitution slots, one
+    // "factors written" flag per node
+    lb.w.queue = c->lu_persistent ? (int*)scratch(c, kSlotQueue, sizeof(int) * (2 + (size_t)nodes)) : nullptr;
     lb.w.lu_variant = c->lu_variant;
     if (!lb.w.ab || !lb.w.ipiv || !lb.w.b0 || !lb.w.c0 || !lb.w.rhs || !lb.w.diag || !lb.w.y || !lb.w.uext ||
         (c->lu_persistent && !lb.w.queue))

@@ -28,6 +28,12 @@ struct LuArgsT {
     // ab + node*ublk_stride + b*16*ublk_cols; entries outside the band are 0
     int ublk_cols = 0;         // 0: LAPACK layout
     size_t ublk_stride = 0;
+    // Fused back-substitution (FP64, persistent mode, blocked U, y given):
+    // per-node flags set once a node's factors are written, a second
+    // dispatch counter (both zeroed by the launch), and x = U^-1 y [nodes][n]
+    int* bs_flags = nullptr;
+    int* bs_queue = nullptr;
+    T* bs_x = nullptr;
 };
 
 // Blocked U geometry for n unknowns and bandwidths kl, ku: columns per block

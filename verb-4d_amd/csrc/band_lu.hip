@@ -277,6 +277,33 @@ struct Lds {
 }  // namespace
 
 __global__ void zero_counter(int* c) { *c = 0; }
+// the counters of the fused launch: LU slots, back-substitution slots, and
+// the per-node "factors written" flags
+__global__ void zero_lu_counters(int* q, int* bq, int* flags, int nflags) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        *q = 0;
+        *bq = 0;
+    }
+    if (i < nflags) flags[i] = 0;
+}
+
+constexpr int kBsThreads = 256;
+constexpr int kXRing = 512;  // x entries kept in LDS: >= kl+ku+32, a power of two
+// row stride of the staged diagonal block and of the partial sums: 17 puts
+// the 16 rows that the triangle's lanes read on different LDS banks
+constexpr int kBsPad = NB + 1;
+constexpr size_t kBsLdsDoubles = kXRing + 2 * NB * kBsPad;
+// waits of a fused back-substitution for its node's factors: s_sleep(32)
+// rounds (about 1 us each); past this the node is reported and skipped
+constexpr int kBsMaxSpins = 1 << 18;
+// U blocks in flight per fused back-substitution: one node per workgroup, so
+// its latency is hidden by prefetch depth (the LU's register budget allows it)
+constexpr int kBsFusedSets = 3;
+template <typename T, bool BLK, int NI, int D>
+__device__ __forceinline__ void backsolve_node(const T* __restrict__ ab_all, const T* __restrict__ y_all,
+                                               T* __restrict__ x_all, int n, int stride, int kl, int ku, int ldab,
+                                               int node, int ublk_cols, size_t ublk_stride, T* sh, int tid);
 
 // (FP32 windows up to 8x14 fit 128 registers: two workgroups per CU)
 template <typename T, int RB, int CB>
@@ -1087,7 +1114,49 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
         ju_prev = ju;
     }
     if (tid == 0 && S.info != 0) raise_pivot(a.status, S.info, node);
+    // fused back-substitution: this node's U blocks and y are published
+    // (every thread's stores made visible device-wide, then the flag)
+    if (a.bs_flags) __threadfence();
     __syncthreads();  // the LDS state is reused by the next node
+    if (a.bs_flags && tid == 0) __hip_atomic_store(&a.bs_flags[node], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if constexpr (sizeof(T) == 8) {
+        // Fused back-substitution: once the node queue is drained, the
+        // workgroup takes x = U^-1 y tasks in the same node order from a
+        // second counter, each after its node's factors are published.
+        // Workgroups that run out of LU nodes early (the last, partial round
+        // of nodes over the CUs) thus solve the nodes finished before.
+        if (a.bs_queue) {
+            static_assert(sizeof(L) >= kBsLdsDoubles * sizeof(double), "back-substitution LDS");
+            __shared__ int bs_slot, bs_skip;
+            T* sh = reinterpret_cast<T*>(lds_raw);
+            const int nx = a.nx, kl = nx + 1, ldab = 3 * kl + 1;
+            for (;;) {
+                __syncthreads();
+                if (tid == 0) bs_slot = atomicAdd(a.bs_queue, 1);
+                __syncthreads();
+                const int k = bs_slot;
+                if (k >= a.qnodes) break;
+                const int node = a.order ? a.order[k] : k;
+                if (a.nR > 0 && !a.include_boundary && (node % a.nR == 0 || node % a.nR == a.nR - 1)) continue;
+                if (a.done && a.done[node]) continue;
+                if (tid == 0) {
+                    int spins = 0;
+                    while (__hip_atomic_load(&a.bs_flags[node], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0 &&
+                           spins <= kBsMaxSpins) {
+                        __builtin_amdgcn_s_sleep(32);
+                        spins++;
+                    }
+                    bs_skip = spins > kBsMaxSpins;
+                    if (bs_skip) raise_error(a.status, VERB_ERR_HIP);
+                }
+                __syncthreads();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                if (bs_skip) continue;
+                backsolve_node<T, true, CB, kBsFusedSets>(a.ab, a.y, a.bs_x, a.nn ? a.nn[node] : a.n, a.n, kl, kl, ldab, node,
+                                               a.ublk_cols, a.ublk_stride, sh, tid);
+            }
+        }
     }
 }
 
@@ -1102,12 +1171,149 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
 // solves the 16x16 triangle (staged in LDS) last row first, broadcasting
 // each x with readlane.
 // ---------------------------------------------------------------------------
-constexpr int kBsThreads = 256;
 constexpr int kBsMaxI = 20;  // staged 16-column groups per thread: (kl+ku+16)/16 <= 20
-constexpr int kXRing = 512;  // x entries kept in LDS: >= kl+ku+32, a power of two
-// row stride of the staged diagonal block and of the partial sums: 17 puts
-// the 16 rows that the triangle's lanes read on different LDS banks
-constexpr int kBsPad = NB + 1;
+
+// One node's x = U^-1 y on the first kBsThreads threads of the workgroup
+// (tid: this thread's index; threads past kBsThreads only take part in the
+// barriers); sh: LDS of bs_lds_doubles doubles. n: the node's unknowns,
+// stride: the per-node vector stride.
+template <typename T, bool BLK, int NI, int D>
+__device__ __forceinline__ void backsolve_node(const T* __restrict__ ab_all, const T* __restrict__ y_all,
+                                               T* __restrict__ x_all, int n, int stride, int kl, int ku, int ldab,
+                                               int node, int ublk_cols, size_t ublk_stride, T* sh, int tid) {
+    const bool act = tid < kBsThreads;
+    const int kv = kl + ku;
+    const int WS = BLK ? ublk_cols : kv + NB;  // columns per block (relative to the block start)
+    const int nI = (WS + NB - 1) / NB;
+    const T* ab = ab_all + (BLK ? (size_t)node * ublk_stride : (size_t)node * ldab * stride);
+    const T* yv = y_all + (size_t)node * stride;
+    T* xo = x_all + (size_t)node * stride;
+    T* x = sh;               // ring of kXRing: x[i] at i % kXRing
+    T* sd = sh + kXRing;     // [NB][NB] diagonal block of the current block row
+    T* part = sd + NB * kBsPad;  // [NB][16] partial sums (rows padded to kBsPad)
+    const int ri = tid & 15, t = tid >> 4;
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(ab), (short)0, ldab * n * (int)sizeof(T), 0x00020000);
+    const int nblk = (n + NB - 1) / NB;
+    (void)rs;
+
+    // this thread's entries of block blk: row jb+ri, columns jb + t + 16i
+    // (and, for the 16 threads of the triangle, y of its row)
+    // (every load is unconditional, from a clamped address, and its value
+    // is used as loaded: a load under a branch, or a select on its value,
+    // makes the compiler wait for all loads in flight at that point, which
+    // would undo the prefetch. Entries of groups i >= nI and y past n are
+    // never used: the partial sums take columns up to hi only, the triangle
+    // rows below n only.)
+    auto issue = [&](int blk, T (&v)[NI], T& yr) {
+        const int jb = NB * blk, r = jb + ri;
+        yr = yv[min(r, n - 1)];
+        const int hi = min(r + kv, n - 1);
+        if constexpr (BLK) {
+            // the block is contiguous and clean: entry (ri, t + 16i) at
+            // 256i + tid
+            const T* src = ab + (size_t)blk * NB * WS;
+            static_for<0, NI>([&](auto ic) {
+                constexpr int i = ic;
+                v[i] = src[NB * NB * (i < nI ? i : 0) + tid];
+            });
+        } else {
+            static_for<0, NI>([&](auto ic) {
+                constexpr int i = ic;
+                const int co = t + NB * i;
+                const int col = jb + co;
+                const bool ok = (i < nI) & (co < WS) & (r < n) & (col >= r) & (col <= hi);
+                const unsigned off = (unsigned)(col * (ldab - 1) + kv + r) * (unsigned)sizeof(T);
+                v[i] = load_entry<T>(rs, ok ? off : kOutside);
+            });
+        }
+    };
+    // one block: partial sums over the band right of it straight from the
+    // registers the block was loaded into, then the triangle
+    auto solve_block = [&](int blk, const T (&v)[NI], T yr) {
+        const int jb = NB * blk;
+        // (branch-free: every x of the band is read and the terms outside it
+        // are fma(0, 0, s), so the LDS reads are issued together instead of
+        // one per branch)
+        {
+            const int r = jb + ri;
+            sd[ri * kBsPad + t] = v[0];  // column jb + t
+            const int hi = (r < n) ? min(r + kv, n - 1) - jb : -1;
+            T sacc = T(0);
+            static_for<1, NI>([&](auto ic) {
+                constexpr int i = ic;
+                const int co = t + NB * i;
+                const T xv = x[(jb + co) & (kXRing - 1)];
+                const bool in = co <= hi;
+                sacc = fma(in ? v[i] : T(0), in ? xv : T(0), sacc);
+            });
+            part[ri * kBsPad + t] = sacc;
+        }
+        __syncthreads();
+        if (tid < 64) {
+            // (the triangle is the serial part of the block: issue priority
+            // over the wavefronts of the other workgroups on the CU)
+            __builtin_amdgcn_s_setprio(3);
+            const int i = tid & 15;
+            const int r = jb + i;
+            const bool live = (tid < NB) & (r < n);
+            T sp = T(0);
+#pragma unroll
+            for (int q = 0; q < 16; q++) sp += part[i * kBsPad + q];
+            T urow[NB];
+#pragma unroll
+            for (int k = 0; k < NB; k++) urow[k] = sd[i * kBsPad + k];
+            const T dv = sd[i * kBsPad + i];
+            T b = live ? yr - sp : T(0);
+            const T rd = live ? T(1) / dv : T(0);
+            // rows past n (last block only) solve to 0 and are not stored;
+            // lane i's b is final once step i is past, so its x is stored
+            // after the loop (the value step i broadcast)
+#pragma unroll
+            for (int k = NB - 1; k >= 0; k--) {
+                const T xk = Prec<T>::readlane(b * rd, k);
+                b = (i < k) ? fma(-urow[k], xk, b) : b;
+            }
+            const T xi = b * rd;
+            if (tid < NB) {
+                x[(jb + i) & (kXRing - 1)] = xi;
+                if (r < n) xo[r] = xi;
+            }
+            __builtin_amdgcn_s_setprio(0);
+        }
+        __syncthreads();
+    };
+
+    const int top = nblk - 1;
+    if (!act) {
+        // threads past kBsThreads: the barriers of every block only
+        for (int b = top; b >= 0; b--) {
+            __syncthreads();
+            __syncthreads();
+        }
+        return;
+    }
+    // U blocks are requested D-1 blocks ahead of their solve into D
+    // register sets used in turn (past block 0, block 0 again: loads that
+    // are never used, so that the sequence has no branch)
+    T v[D][NI];
+    T yq[D];
+    static_for<0, D - 1>([&](auto sc) {
+        constexpr int sset = sc;
+        issue(max(top - sset, 0), v[sset], yq[sset]);
+    });
+    for (int blk = top; blk >= 0; blk -= D) {
+        static_for<0, D>([&](auto kc) {
+            constexpr int k = kc;
+            constexpr int nxt = (k + D - 1) % D;  // the set freed by the previous solve
+            const int b = blk - k;
+            if (b < 0) return;
+            issue(max(b - (D - 1), 0), v[nxt], yq[nxt]);
+            solve_block(b, v[k], yq[k]);
+        });
+    }
+}
+
 
 template <typename T, bool BLK, int NI, int D>
 __global__ __launch_bounds__(kBsThreads) void band_backsolve(const T* __restrict__ ab_all,
@@ -1121,130 +1327,9 @@ __global__ __launch_bounds__(kBsThreads) void band_backsolve(const T* __restrict
     const int node = order ? order[blockIdx.x] : blockIdx.x;
     if (nR > 0 && !include_boundary && (node % nR == 0 || node % nR == nR - 1)) return;
     if (done && done[node]) return;
-    const int stride = n;
-    if (nn) n = nn[node];
-    const int kv = kl + ku;
-    const int WS = BLK ? ublk_cols : kv + NB;  // columns per block (relative to the block start)
-    const int nI = (WS + NB - 1) / NB;
-    const T* ab = ab_all + (BLK ? (size_t)node * ublk_stride : (size_t)node * ldab * stride);
-    const T* yv = y_all + (size_t)node * stride;
-    T* xo = x_all + (size_t)node * stride;
     extern __shared__ double sh_raw[];  // 8-byte aligned
-    T* sh = reinterpret_cast<T*>(sh_raw);
-    T* x = sh;               // ring of kXRing: x[i] at i % kXRing
-    T* sd = sh + kXRing;     // [NB][NB] diagonal block of the current block row
-    T* part = sd + NB * kBsPad;  // [NB][16] partial sums (rows padded to kBsPad)
-    const int tid = threadIdx.x;
-    const int ri = tid & 15, t = tid >> 4;
-    const __amdgpu_buffer_rsrc_t rs =
-        __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(ab), (short)0, ldab * n * (int)sizeof(T), 0x00020000);
-    const int nblk = (n + NB - 1) / NB;
-    (void)rs;
-
-    // this thread's entries of block blk: row jb+ri, columns jb + t + 16i
-    // (and, for the 16 threads of the triangle, y of its row)
-    auto issue = [&](int blk, T (&v)[NI], T& yr) {
-        const int jb = NB * blk, r = jb + ri;
-        if (tid < NB) yr = (r < n) ? yv[r] : T(0);
-        const int hi = min(r + kv, n - 1);
-        if constexpr (BLK) {
-            // the block is contiguous and clean: entry (ri, t + 16i) at
-            // 256i + tid
-            const T* src = ab + (size_t)blk * NB * WS;
-            static_for<0, NI>([&](auto ic) {
-                constexpr int i = ic;
-                v[i] = (i < nI) ? src[NB * NB * i + tid] : T(0);
-            });
-        } else {
-            static_for<0, NI>([&](auto ic) {
-                constexpr int i = ic;
-                if (i < nI) {
-                    const int co = t + NB * i;
-                    const int col = jb + co;
-                    const bool ok = (co < WS) & (r < n) & (col >= r) & (col <= hi);
-                    const unsigned off = (unsigned)(col * (ldab - 1) + kv + r) * (unsigned)sizeof(T);
-                    v[i] = load_entry<T>(rs, ok ? off : kOutside);
-                } else {
-                    v[i] = T(0);
-                }
-            });
-        }
-    };
-    // one block: partial sums over the band right of it straight from the
-    // registers the block was loaded into, then the triangle
-    auto solve_block = [&](int blk, const T (&v)[NI], T yr) {
-        const int jb = NB * blk;
-        {
-            const int r = jb + ri;
-            sd[ri * kBsPad + t] = v[0];  // column jb + t
-            const int hi = (r < n) ? min(r + kv, n - 1) - jb : -1;
-            T sacc = T(0);
-            static_for<1, NI>([&](auto ic) {
-                constexpr int i = ic;
-                const int co = t + NB * i;
-                const T xv = x[(jb + co) & (kXRing - 1)];
-                sacc = (co <= hi) ? fma(v[i], xv, sacc) : sacc;
-            });
-            part[ri * kBsPad + t] = sacc;
-        }
-        __syncthreads();
-        if (tid < 64) {
-            // (the triangle is the serial part of the block: issue priority
-            // over the wavefronts of the other workgroups on the CU)
-            __builtin_amdgcn_s_setprio(3);
-            const int i = tid & 15;
-            const int r = jb + i;
-            const bool live = (tid < NB) & (r < n);
-            T b = T(0), rd = T(0);
-            T urow[NB];
-#pragma unroll
-            for (int k = 0; k < NB; k++) urow[k] = T(0);
-            if (live) {
-                T sp = T(0);
-#pragma unroll
-                for (int q = 0; q < 16; q++) sp += part[i * kBsPad + q];
-                b = yr - sp;
-#pragma unroll
-                for (int k = 0; k < NB; k++) urow[k] = sd[i * kBsPad + k];
-                rd = T(1) / sd[i * kBsPad + i];
-            }
-#pragma unroll
-            for (int k = NB - 1; k >= 0; k--) {
-                if (jb + k < n) {
-                    const T xk = Prec<T>::readlane(b * rd, k);
-                    if (tid == k) {
-                        x[(jb + k) & (kXRing - 1)] = xk;
-                        xo[jb + k] = xk;
-                    }
-                    b = (i < k) ? fma(-urow[k], xk, b) : b;
-                }
-            }
-            __builtin_amdgcn_s_setprio(0);
-        }
-        __syncthreads();
-    };
-
-    // U blocks are requested D-1 blocks ahead of their solve into D
-    // register sets used in turn
-    T v[D][NI];
-    T yq[D];
-    const int top = nblk - 1;
-    static_for<0, D - 1>([&](auto sc) {
-        constexpr int sset = sc;
-        yq[sset] = T(0);
-        if (top - sset >= 0) issue(top - sset, v[sset], yq[sset]);
-    });
-    yq[D - 1] = T(0);
-    for (int blk = top; blk >= 0; blk -= D) {
-        static_for<0, D>([&](auto kc) {
-            constexpr int k = kc;
-            constexpr int nxt = (k + D - 1) % D;  // the set freed by the previous solve
-            const int b = blk - k;
-            if (b < 0) return;
-            if (b - (D - 1) >= 0) issue(b - (D - 1), v[nxt], yq[nxt]);
-            solve_block(b, v[k], yq[k]);
-        });
-    }
+    backsolve_node<T, BLK, NI, D>(ab_all, y_all, x_all, nn ? nn[node] : n, n, kl, ku, ldab, node, ublk_cols,
+                                  ublk_stride, reinterpret_cast<T*>(sh_raw), threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1475,7 +1560,32 @@ int launch_band_lu_t(const LuArgsT<T>& a, int nodes, hipStream_t stream) {
         int cus = 0;
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         grid = min(nodes, max(1, cus));
-        hipLaunchKernelGGL(zero_counter, dim3(1), dim3(1), 0, stream, a.queue);
+        if (a.bs_queue && RB * 64 < kBsThreads) {
+            // too few threads for the back-substitution in the same kernel:
+            // the LU, then the back-substitution kernel
+            LuArgsT<T> b = a;
+            b.bs_queue = nullptr;
+            b.bs_flags = nullptr;
+            b.bs_x = nullptr;
+            int rc = launch_band_lu_t<T, RB, CB>(b, nodes, stream);
+            if (rc) return rc;
+            if constexpr (sizeof(T) == 8)
+                return launch_backsolve(a.ab, a.y, nullptr, a.bs_x, a.n, a.nx + 1, a.nx + 1, 3 * (a.nx + 1) + 1,
+                                        nodes, a.nR, a.include_boundary, stream, a.nn, a.order, a.done, a.ublk_cols,
+                                        a.ublk_stride);
+            return VERB_ERR_ARG;
+        }
+        if (a.bs_queue) {
+            if (sizeof(T) != 8 || !a.bs_flags || !a.bs_x || !a.y || a.write_L || a.ublk_cols <= 0 ||
+                a.ublk_cols > CB * NB)
+                return VERB_ERR_ARG;
+            hipLaunchKernelGGL(zero_lu_counters, dim3((nodes + 255) / 256), dim3(256), 0, stream, a.queue, a.bs_queue,
+                               a.bs_flags, nodes);
+        } else {
+            hipLaunchKernelGGL(zero_counter, dim3(1), dim3(1), 0, stream, a.queue);
+        }
+    } else if (a.bs_queue) {
+        return VERB_ERR_ARG;
     }
     hipLaunchKernelGGL((band_lu_window<T, RB, CB>), dim3(grid), dim3(RB * 64), lds, stream, a);
     return hipGetLastError() == hipSuccess ? VERB_OK : VERB_ERR_HIP;

@@ -606,15 +606,25 @@ int launch_local(const verb_grid& g, double* psd, const LocalWork& w, bool facto
         // U goes out in 16-row blocks (the factors are not kept), which the
         // back-substitution streams
         if (!ublk_geometry(n, kl, ku, &la.ublk_cols, &la.ublk_stride)) la.ublk_cols = 0;
+        // persistent launch with blocked U: the back-substitution runs in the
+        // same kernel, on the workgroups that run out of nodes to factor
+        const bool fused = w.queue && la.ublk_cols > 0;
+        if (fused) {
+            la.bs_queue = w.queue + 1;
+            la.bs_flags = w.queue + 2;
+            la.bs_x = w.rhs;
+        }
         int rc = launch_band_lu(la, nodes, stream, w.lu_variant);
         if (rc) return rc;
         if (marks) {
             hipEventRecord(marks[3], stream);
             hipEventRecord(marks[4], stream);
         }
-        rc = launch_backsolve(w.ab, w.y, w.uext, w.rhs, n, kl, ku, ldab, nodes, g.nR, g.include_boundary, stream, nns,
-                              order, nullptr, la.ublk_cols, la.ublk_stride);
-        if (rc) return rc;
+        if (!fused) {
+            rc = launch_backsolve(w.ab, w.y, w.uext, w.rhs, n, kl, ku, ldab, nodes, g.nR, g.include_boundary, stream,
+                                  nns, order, nullptr, la.ublk_cols, la.ublk_stride);
+            if (rc) return rc;
+        }
         hipLaunchKernelGGL(scatter_psd, agrid, dim3(kAsmThreads), 0, stream, g, w.rhs, psd, k0s);
         if (marks) hipEventRecord(marks[5], stream);
         return hipGetLastError() == hipSuccess ? VERB_OK : VERB_ERR_HIP;
