@@ -164,3 +164,39 @@ def test_da_convection_no_observations_singular(ctx):
         dg.da_convection(dev(obs), dev(loss), cov, model, 0.0255, 2.0, 1e-2, 4e-2, plane_begin=q, plane_count=1)
     assert e.value.status == -11
     dg.check_status()
+
+
+def test_da_convection_handle_survives_graph_captures():
+    """The filter's rocBLAS handle lives in the context from the first
+    verb_da_convection call until verb_ctx_destroy: step graphs captured in
+    between (here two, for two grids) leave it alone, so a later filter call
+    gives the same result as the first one from the same inputs, and the
+    context is destroyed once."""
+    import torch
+    from verb4d.device import DeviceGrid
+    w, loss, satd, obs = _setup(153)
+    NQ, N = w.nV * w.nK, w.nP * w.nR
+    q0, nq = 100, 6
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    tobs, tloss, tsat = dev(obs), dev(loss), dev(satd)
+    c = verb4d.Context()
+    dg = DeviceGrid(w, ctx=c)
+    w2 = W.config0(seed=154)
+    w2.psd0 = w2.psd0 * 0.5
+    dg2 = DeviceGrid(w2, ctx=c)
+    results = []
+    for rep in range(2):
+        cov = torch.zeros((NQ, N, N), dtype=torch.float64, device="cuda")
+        model = torch.zeros((NQ, N, N), dtype=torch.float64, device="cuda")
+        dg.set_psd(w.psd0)
+        dg.da_convection(tobs, tloss, cov, model, 0.0255, 2.0, 1e-2, 4e-2, saturation=tsat, plane_begin=q0,
+                         plane_count=nq)
+        dg.check_status()
+        results.append((dg.psd_host(), cov.cpu().numpy()))
+        if rep == 0:
+            dg.step_graph()  # capture for this grid
+            dg2.step_graph()  # a different grid: a second capture
+            dg.check_status()
+    assert np.array_equal(results[0][0], results[1][0])
+    assert np.array_equal(results[0][1], results[1][1])
+    c.close()

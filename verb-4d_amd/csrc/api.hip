@@ -219,7 +219,7 @@ int verb_ctx_destroy(verb_ctx* c) {
     hipStreamSynchronize(c->stream);
     if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
     if (c->graph) hipGraphDestroy(c->graph);
-    verb::da_conv_release(c->blas);
+    verb::da_conv_release(&c->blas);
     for (auto& b : c->bufs)
         if (b.first) hipFree(b.first);
     hipFree(c->status);
@@ -781,7 +781,6 @@ int verb_step_graph(verb_ctx* c, const verb_grid* g, double* psd, const int* k0)
     if (rc) return rc;
     if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
     if (c->graph) hipGraphDestroy(c->graph);
-    verb::da_conv_release(c->blas);
     c->graph_exec = nullptr;
     c->graph = nullptr;
     const bool saved_timing = c->timing;

@@ -159,7 +159,9 @@ int launch_da_radial(const verb_grid& g, double* psd, const double* obs, double*
 int run_da_convection(verb_ctx* c, void** blas_slot, hipStream_t st, int* status, const verb_grid& g, double* psd,
                       const double* obs, const double* loss, const double* satd, double* cov, double* model,
                       const verb_da_conv_params& p);
-void da_conv_release(void* blas);
+// destroys the handle that run_da_convection created in *blas_slot (if any)
+// and clears the slot
+void da_conv_release(void** blas_slot);
 // a context's grow-only scratch slot (api.hip)
 void* ctx_scratch(verb_ctx* c, int slot, size_t bytes);
 

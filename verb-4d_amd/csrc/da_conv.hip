@@ -366,8 +366,9 @@ rocblas_status power_rm(rocblas_handle h, hipStream_t st, const double* A, int p
 
 }  // namespace
 
-void da_conv_release(void* blas) {
-    Blas* b = (Blas*)blas;
+void da_conv_release(void** blas_slot) {
+    Blas* b = (Blas*)*blas_slot;
+    *blas_slot = nullptr;
     if (!b) return;
     if (b->h) rocblas_destroy_handle(b->h);
     delete b;
