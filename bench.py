@@ -65,12 +65,24 @@ def lu_flops(n, kl, ku):
     return float(np.sum(l + 2 * l * u))
 
 
-def phase_work(w):
-    """Algorithmic work per launch of each kernel phase: (amount, unit, bound)."""
+def phase_work(w, fused):
+    """Algorithmic work per launch of each kernel phase: (amount, unit, bound).
+    fused: the back-substitution runs inside the persistent LU kernel (the
+    local_factor phase), and the local_solve phase is the PSD scatter only."""
     n = w.nV * w.nK
     kl = ku = w.nV + 1
     nodes = w.nP * w.nR
     pts = w.points
+    if fused:
+        return {
+            "convection": (56.0 * pts, "B", "hbm"),
+            "radial": (56.0 * pts, "B", "hbm"),
+            "local_assemble": (88.0 * pts, "B", "hbm"),
+            # LU flops; the same launch also streams U back for x = U^-1 y
+            "local_factor": (lu_flops(n, kl, ku) * nodes, "flop", "mfma"),
+            # x read, PSD written
+            "local_solve": (16.0 * pts, "B", "hbm"),
+        }
     return {
         # PSD read+write, VP, VR, G_conv, losses, sources: 7 doubles / point
         "convection": (56.0 * pts, "B", "hbm"),
@@ -486,7 +498,10 @@ def main():
         print("bench: %d graph-replayed steps, %.3f ms/step, wall %.3f s" % (args.steps, ms_per_step, wall),
               file=sys.stderr, flush=True)
 
-    work = phase_work(w)
+    # the autotuner's persistent LU queue runs the back-substitution in the
+    # same launch
+    fused = bool((tuning or {}).get("lu_persistent", 0))
+    work = phase_work(w, fused)
     phase_out = {}
     dom, dom_ms = None, -1.0
     for p, (ms, cnt) in phases.items():
@@ -507,6 +522,10 @@ def main():
     roofline = {"bound": bound, "achieved": d["achieved"],
                 "peak": PEAK_FP64_TFLOPS if unit == "flop" else PEAK_HBM_GBS, "unit": d["unit"],
                 "frac": d["frac"], "traffic": profiled_traffic(dom), "kernel": dom}
+    if fused and dom == "local_factor":
+        roofline["note"] = ("band_lu_window with the back-substitution fused in (the persistent launch solves "
+                            "x = U^-1 y on the workgroups that run out of nodes to factor); achieved = LU flops / "
+                            "launch time")
 
     cached = None
     if not args.no_cached:
