@@ -381,6 +381,9 @@ typedef struct {
     int lu_persistent;    /* 1: one LU workgroup per CU takes nodes from a queue */
     double convection_ms; /* verb_autotune: time of the chosen convection launch */
     double local_ms;      /* verb_autotune: time of the chosen local-diffusion launches */
+    int convection_kernel; /* 0: a lane per (MLT or R) line, the line in registers, where the
+                              build has that kernel for (nP, nR) (24x29, 24x10), else 1;
+                              1: a lane per grid point over LDS tiles */
 } verb_tuning;
 
 int verb_set_tuning(verb_ctx* ctx, const verb_tuning* t);

@@ -42,6 +42,7 @@ struct verb_ctx {
     long counts[VERB_NUM_PHASES] = {0};
     // launch shapes (verb_set_tuning / verb_autotune)
     int conv_tq = 0;
+    int conv_kernel = 0;  // 0: line kernel where instantiated, 1: tile kernel
     int lu_variant = -1;
     int lu_persistent = 0;
     // captured plain step (verb_step_graph) and what it was captured for
@@ -49,7 +50,7 @@ struct verb_ctx {
     hipGraphExec_t graph_exec = nullptr;
     verb_grid graph_grid{};
     double* graph_psd = nullptr;
-    int graph_tune[3] = {0, -1, 0};
+    int graph_tune[4] = {0, -1, 0, 0};
     std::vector<int> graph_k0;  // pitch-angle extents the graph was captured for (empty: uniform)
     unsigned long long graph_gen = 0;  // scratch_gen at capture
     // the uneven extents last uploaded (first column, unknowns, dispatch order)
@@ -292,7 +293,7 @@ int verb_convection_batched_ex(verb_ctx* c, const verb_grid* g, double* psd, con
     std::lock_guard<std::mutex> lk(c->mu);
     const int m = mark_begin(c, VERB_PHASE_CONVECTION);
     int rc = verb::launch_convection(*g, psd, c->status, c->stream, c->conv_tq, opt ? opt->psd_lost : nullptr,
-                                     opt ? opt->fast_convection : 0);
+                                     opt ? opt->fast_convection : 0, c->conv_kernel);
     mark_end(c, m);
     return rc ? set_err(c, rc, "convection launch rejected") : VERB_OK;
 }
@@ -653,8 +654,11 @@ int verb_set_tuning(verb_ctx* c, const verb_tuning* t) {
         if (variant < 0) return set_err(c, VERB_ERR_ARG, "no register-window LU of that shape");
     }
     if (t->convection_tq < 0 || t->convection_tq > 64) return set_err(c, VERB_ERR_ARG, "convection_tq out of range");
+    if (t->convection_kernel < 0 || t->convection_kernel > 1)
+        return set_err(c, VERB_ERR_ARG, "convection_kernel must be 0 or 1");
     std::lock_guard<std::mutex> lk(c->mu);
     c->conv_tq = t->convection_tq;
+    c->conv_kernel = t->convection_kernel;
     c->lu_variant = variant;
     c->lu_persistent = t->lu_persistent ? 1 : 0;
     return VERB_OK;
@@ -664,6 +668,7 @@ int verb_get_tuning(verb_ctx* c, verb_tuning* t) {
     if (!c || !t) return VERB_ERR_ARG;
     std::lock_guard<std::mutex> lk(c->mu);
     t->convection_tq = c->conv_tq;
+    t->convection_kernel = c->conv_kernel;
     t->lu_rb = t->lu_cb = 0;
     if (c->lu_variant >= 0) verb::lu_variant_shape(c->lu_variant, 0, &t->lu_rb, &t->lu_cb);
     t->lu_persistent = c->lu_persistent;
@@ -712,17 +717,25 @@ int verb_autotune(verb_ctx* c, const verb_grid* g, double* psd, verb_tuning* out
     // convection: planes per workgroup
     best.convection_ms = -1.0;
     if (3 < g->nP || 3 < g->nR) {
-        for (int tq : {1, 2, 4, 8}) {
-            c->conv_tq = tq;
-            time_once(c, psd, save, bytes, [&] { return verb_convection_batched(c, g, psd); });  // warm-up
-            const double ms = time_once(c, psd, save, bytes, [&] { return verb_convection_batched(c, g, psd); });
-            if (ms >= 0.0 && (best.convection_ms < 0.0 || ms < best.convection_ms)) {
-                best.convection_ms = ms;
-                best.convection_tq = tq;
+        // the line kernel (where it is built for this grid; otherwise these
+        // trials run the tile kernel too) and the tile kernel
+        for (int kern = 0; kern <= 1; kern++) {
+            for (int tq : {1, 2, 4, 8, 16}) {
+                if (kern == 1 && tq > 8) continue;
+                c->conv_kernel = kern;
+                c->conv_tq = tq;
+                time_once(c, psd, save, bytes, [&] { return verb_convection_batched(c, g, psd); });  // warm-up
+                const double ms = time_once(c, psd, save, bytes, [&] { return verb_convection_batched(c, g, psd); });
+                if (ms >= 0.0 && (best.convection_ms < 0.0 || ms < best.convection_ms)) {
+                    best.convection_ms = ms;
+                    best.convection_tq = tq;
+                    best.convection_kernel = kern;
+                }
             }
         }
     }
     c->conv_tq = best.convection_tq;
+    c->conv_kernel = best.convection_kernel;
     // local diffusion: register-window shape and persistent dispatch
     best.local_ms = -1.0;
     int best_variant = -1, best_pers = 0;
@@ -759,7 +772,7 @@ int verb_autotune(verb_ctx* c, const verb_grid* g, double* psd, verb_tuning* out
 
 int verb_step_graph(verb_ctx* c, const verb_grid* g, double* psd, const int* k0) {
     if (!c || !grid_ok(g) || !psd) return VERB_ERR_ARG;
-    const int tune[3] = {c->conv_tq, c->lu_variant, c->lu_persistent};
+    const int tune[4] = {c->conv_tq, c->lu_variant, c->lu_persistent, c->conv_kernel};
     std::vector<int> key;
     if (k0) key.assign(k0, k0 + g->nP * g->nR);
     // the captured kernels hold scratch addresses and read the extent tables:

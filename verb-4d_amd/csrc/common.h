@@ -27,14 +27,15 @@ int ctx_error(verb_ctx* c, int code, const char* what);
 
 // Launchers (defined in the .hip files). All asynchronous on `stream`.
 int launch_radial(const verb_grid& g, double* psd, int* status, hipStream_t stream);
-// tq: (V,K) planes per workgroup (0: the largest of 4, 2, 1 that fits in LDS);
-// lost (nullable, [P][R][V][K]): SAVE_PSD_LOST_CONV accumulation; fast:
-// FAST_CONVECTION outer sub-step rule.
+// tq: (V,K) planes per workgroup (0: automatic); lost (nullable,
+// [P][R][V][K]): SAVE_PSD_LOST_CONV accumulation; fast: FAST_CONVECTION outer
+// sub-step rule; kernel: 0 = the line kernel where it is instantiated for
+// (nP, nR), else the tile kernel; 1 = the tile kernel.
 int launch_convection(const verb_grid& g, double* psd, int* status, hipStream_t stream, int tq = 0,
-                      double* lost = nullptr, int fast = 0);
-// single_outer: one outer step (a bare QUICKEST6 line)
+                      double* lost = nullptr, int fast = 0, int kernel = 0);
+// single_outer: one outer step (a bare QUICKEST6 line; tile kernel)
 int launch_convection_mode(const verb_grid& g, double* psd, int* status, hipStream_t stream, int single_outer,
-                           int tq, double* lost = nullptr, int fast = 0);
+                           int tq, double* lost = nullptr, int fast = 0, int kernel = 0);
 
 struct LocalWork {
     double* ab;       // band factors, per node (2kl+ku+1)*n doubles (LAPACK layout)

@@ -19,6 +19,8 @@
 //   (d) conservative update, with the boundary fix-ups folded into (b)/(d).
 // Results ping-pong between two LDS buffers, so no pass reads a value
 // another lane of the same pass writes.
+#include <utility>
+
 #include "common.h"
 
 namespace verb {
@@ -621,6 +623,397 @@ __global__ __launch_bounds__(kThreads) void convection_kernel(const verb_grid g,
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Line kernel: one lane per line of one plane, the line in registers.
+//
+// A workgroup owns TQ consecutive (V,K) planes, kept in LDS ([p][r][tq]) for
+// the whole call as above, and 32 lanes per plane. For the MLT sweep lane li
+// takes the line r = li (points p = 0..NP-1); for the R sweep, the G_conv
+// weighting and the losses and sources, the line p = li (points r =
+// 0..NR-1). A lane loads its line from LDS once per sweep, runs every inner
+// QUICKEST step on it in registers (the four passes become four unrolled
+// loops over the line with compile-time indices, ghosts included), and
+// stores it back: two barriers per outer sub-step, one between the sweeps
+// and one after the second. Line lengths are template arguments.
+// ---------------------------------------------------------------------------
+template <int B, class Fn, int... I>
+__device__ inline void sfor_seq(Fn&& f, std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, B + I>{}), ...);
+}
+template <int B, int E, class Fn>
+__device__ inline void sfor(Fn&& f) {
+    if constexpr (E > B) sfor_seq<B>(f, std::make_integer_sequence<int, E - B>{});
+}
+
+// Point J (compile time; a ghost outside 0..N-1) of the line xr with the
+// ghost rules of QUICKEST6.cpp:89-143 (orig_at).
+template <int N, int J>
+__device__ inline double xo(const double (&xr)[N], const LineBC& bc) {
+    if constexpr (J >= 0 && J < N) {
+        return xr[J];
+    } else if constexpr (J < 0) {
+        if (bc.lt == VERB_BC_PERIODIC) return xr[J + N - 1];
+        if (bc.lt == VERB_BC_CONSTANT_VALUE) return bc.lbc;
+        double v = xr[0] - bc.sl;
+#pragma unroll
+        for (int ig = 2; ig <= -J; ig++) v = v - bc.sl;
+        return v;
+    } else {
+        constexpr int m = J - (N - 1);
+        if (bc.ut == VERB_BC_PERIODIC) return xr[m];
+        if (bc.ut == VERB_BC_CONSTANT_VALUE) return bc.ubc;
+        double v = xr[N - 1] + bc.su;
+#pragma unroll
+        for (int ig = 2; ig <= m; ig++) v = v + bc.su;
+        return v;
+    }
+}
+
+// Point J of the ghost-extended unlimited solution (unl_at): lower ghosts
+// from the pre-step values xr, the line and its upper ghosts from yr.
+template <int N, int J>
+__device__ inline double xu(const double (&xr)[N], const double (&yr)[N], const LineBC& bc) {
+    if constexpr (J < 0) {
+        return xo<N, J>(xr, bc);
+    } else if constexpr (J < N) {
+        return yr[J];
+    } else {
+        constexpr int m = J - (N - 1);
+        if (bc.ut == VERB_BC_PERIODIC) return yr[m];
+        if (bc.ut == VERB_BC_CONSTANT_VALUE) return bc.ubc;
+        double v = yr[N - 1] + bc.su;
+#pragma unroll
+        for (int ig = 2; ig <= m; ig++) v = v + bc.su;
+        return v;
+    }
+}
+
+// Updated value of point K from face values A (face i left of point i) and
+// face Courant numbers fc: QUICKEST6's conservative update with its boundary
+// rules (:257-272, 457-477): a constant-value end takes its value, a
+// constant-derivative end the update of its neighbour. (The periodic last
+// point takes point 0's update; the caller copies it.)
+template <int N, int K>
+__device__ inline double upd_point(const double (&xr)[N], const double (&fc)[N], const double (&A)[N],
+                                   const LineBC& bc) {
+    if constexpr (K > 0 && K < N - 1) {
+        return xr[K] - (fc[K + 1] * A[K + 1] - fc[K] * A[K]);
+    } else if constexpr (K == 0) {
+        if (bc.lt == VERB_BC_CONSTANT_VALUE) return bc.lbc;
+        if (bc.lt == VERB_BC_CONSTANT_DERIVATIVE) return xr[1] - (fc[2] * A[2] - fc[1] * A[1]);
+        return xr[0] - (fc[1] * A[1] - fc[0] * A[0]);
+    } else {
+        if (bc.ut == VERB_BC_CONSTANT_VALUE) return bc.ubc;
+        return xr[N - 2] - (fc[N - 1] * A[N - 1] - fc[N - 2] * A[N - 2]);
+    }
+}
+
+// One QUICKEST step on the line xr (N points, in registers), the result
+// written to out[i * os] (LDS). The four passes of the reference's step
+// (unlimited faces, unlimited values, limited faces, final update) run as
+// one pipeline over the line, each pass a fixed number of points behind
+// the one it reads, so only a window of each stays live: at stage s, the
+// Courant number and unlimited value of face s+6, the unlimited value of
+// point s+4, the limited value of face s and the final value of point s-2.
+// Face i is left of point i (face 0 exists on periodic lines only); the
+// Courant number of face i is (C[i-1] + C[i]) / 2 with C[i] = V[i] / dx *
+// dt_inner the point's Courant number (QUICKEST6.cpp:148-161).
+template <int N>
+__device__ inline void line_step(const double (&xr)[N], const double (&C)[N], const LineBC& bc, double* out,
+                                 int os) {
+    // (addresses formed where they are used, not hoisted and kept live)
+    asm volatile("" : "+v"(out), "+s"(os));
+    const bool per = bc.lt == VERB_BC_PERIODIC;
+    double fc[N], F[N], FL[N], y[N];
+    double out0 = 0.0;  // point 0's final value (the periodic last point takes it)
+    sfor<-6, N + 2>([&](auto sc) {
+        constexpr int s = sc;
+        constexpr int kf = s + 6;
+        if constexpr (kf >= 0 && kf < N) {
+            constexpr int ka = (kf == 0) ? N - 2 : kf - 1;
+            fc[kf] = 0.0;
+            F[kf] = 0.0;
+            if (kf > 0 || per) {
+                fc[kf] = (C[ka] + C[kf]) / 2;
+                double W[10];
+                sfor<0, 10>([&](auto kc) {
+                    constexpr int k = kc;
+                    W[k] = xo<N, kf - 5 + k>(xr, bc);
+                });
+                F[kf] = unlimited_face(W, fc[kf]);
+            }
+        }
+        constexpr int ky = s + 4;
+        if constexpr (ky >= 0 && ky < N - 1) y[ky] = upd_point<N, ky>(xr, fc, F, bc);
+        if constexpr (ky == N - 1) y[ky] = (bc.ut == VERB_BC_PERIODIC) ? y[0] : upd_point<N, ky>(xr, fc, F, bc);
+        if constexpr (s >= 0 && s < N) {
+            FL[s] = 0.0;
+            if (s > 0 || per) {
+                const double cr = fc[s];
+                double UW[10];
+                sfor<0, 10>([&](auto kc) {
+                    constexpr int k = kc;
+                    UW[k] = xu<N, s - 5 + k>(xr, y, bc);
+                });
+                // original points s-2 .. s+1: (ou, oc, od) is (s-2, s-1, s)
+                // for a non-negative Courant number, else (s+1, s, s-1)
+                const double o0 = xo<N, s - 2>(xr, bc), o1 = xo<N, s - 1>(xr, bc);
+                const double o2 = xo<N, s>(xr, bc), o3 = xo<N, s + 1>(xr, bc);
+                const double ou = (cr >= 0) ? o0 : o3;
+                const double oc = (cr >= 0) ? o1 : o2;
+                const double od = (cr >= 0) ? o2 : o1;
+                FL[s] = limited_face(UW, ou, oc, od, F[s], cr);
+            }
+        }
+        constexpr int ko = s - 2;
+        if constexpr (ko == 0) {
+            out0 = upd_point<N, 0>(xr, fc, FL, bc);
+            out[0] = out0;
+        }
+        if constexpr (ko > 0 && ko < N - 1) out[ko * os] = upd_point<N, ko>(xr, fc, FL, bc);
+        if constexpr (ko == N - 1)
+            out[ko * os] = (bc.ut == VERB_BC_PERIODIC) ? out0 : upd_point<N, ko>(xr, fc, FL, bc);
+        // (one stage at a time: the windows above are all that stays live)
+        __builtin_amdgcn_sched_barrier(0);
+    });
+}
+
+// Point Courant numbers C[i] = V[i] / dx * dti of a line, V[i] at
+// vel[i * vs]. Called once per QUICKEST step: the base pointer goes through
+// an empty asm, so that neither the line's addresses nor the Courant numbers
+// and the face polynomial coefficients that depend on them alone are
+// hoisted out of the sub-step loops and kept live for the whole line.
+template <int N>
+__device__ inline void line_courant(const double* vel, int vs, double dx, double dti, double (&C)[N]) {
+    asm volatile("" : "+v"(vel));
+    sfor<0, N>([&](auto ic) {
+        constexpr int i = ic;
+        C[i] = vel[i * vs] / dx * dti;
+    });
+}
+
+constexpr int kLineSlots = 32;  // lanes per plane (>= nP, nR)
+
+// Inner sub-step count of a line whose largest |velocity| is umax
+// (QUICKEST6.cpp:37-52); 0 when it cannot run (reported).
+__device__ inline int inner_steps(double dtq, double umax, double dx, int* status) {
+    int nin = 0;
+    const double sd = dtq * umax / dx / 1.0;
+    if (sd <= 1) nin = 1;
+    else if (sd <= kMaxSubsteps) nin = (int)ceil(sd);
+    else if (sd > kMaxSubsteps) raise_error(status, VERB_ERR_COURANT);  // incl. infinite velocity
+    if (nin > 0) {
+        const double dti = dtq / nin;
+        if (fabs(umax / dx * dti) > 1) raise_error(status, VERB_ERR_COURANT);
+    }
+    return nin;
+}
+
+template <int NP, int NR>
+__global__ __launch_bounds__(512) void convection_lines(const verb_grid g, double* __restrict__ psd,
+                                                        int* __restrict__ status, int TQ, int ntiles, int per_xcd,
+                                                        double* __restrict__ lost, int fast) {
+    static_assert(NP <= kLineSlots && NR <= kLineSlots && NP >= 6 && NR >= 6, "line lengths");
+    // XCD-aware tile order, as convection_kernel
+    const int bid = blockIdx.x;
+    const int tile = (bid % 8) * per_xcd + bid / 8;
+    if (tile >= ntiles) return;
+
+    extern __shared__ double lds[];
+    const int tid = threadIdx.x;
+    const int tqs = __builtin_ctz((unsigned)TQ);
+    const int nthreads = TQ * kLineSlots;
+    const int npt = NP * NR * TQ;
+    const long NQ = (long)g.nV * g.nK;
+    double* X = lds;                         // [p][r][tq]
+    double* redP = X + npt;                  // [tq][line] max |VP| of MLT line r
+    double* redR = redP + TQ * kLineSlots;   // [tq][line] max |VR| of R line p
+    double* dtq_s = redR + TQ * kLineSlots;  // [tq]
+    int* ns_s = (int*)(dtq_s + TQ);          // [tq]
+    Tile T;
+    T.nP = NP; T.nR = NR; T.TQ = TQ; T.npt = npt; T.tq_shift = tqs; T.mR = 0u;
+    T.NQ = NQ;
+    T.q0 = (long)tile * TQ;
+    T.bc = nullptr;
+
+    const int tq = tid & (TQ - 1), li = tid >> tqs;
+    const long q = T.q0 + tq;
+    const bool qok = q < NQ;
+    const size_t pstride = (size_t)NR * NQ;  // global stride between MLT points
+
+    for (int t = tid; t < npt; t += nthreads) {
+        const int tq2 = t & (TQ - 1);
+        const int pr = t >> tqs;
+        const int p = pr / NR, r = pr - p * NR;
+        const long qq = T.q0 + tq2;
+        X[t] = (qq < NQ) ? psd[((size_t)p * NR + r) * NQ + qq] : 0.0;
+    }
+    // largest |VP| of MLT line r = li (every r: the plane's maximum) and |VR|
+    // of R line p = li
+    double umP = 0.0, umR = 0.0;
+    if (qok && li < NR) {
+        const double* v = g.VP + (size_t)li * NQ + q;
+#pragma unroll
+        for (int p = 0; p < NP; p++) umP = fmax(umP, fabs(v[p * pstride]));
+    }
+    if (qok && li < NP) {
+        const double* v = g.VR + (size_t)li * pstride + q;
+#pragma unroll
+        for (int r = 0; r < NR; r++) umR = fmax(umR, fabs(v[(size_t)r * NQ]));
+    }
+    redP[tq * kLineSlots + li] = umP;
+    redR[tq * kLineSlots + li] = umR;
+    __syncthreads();
+    if (tid < TQ) {
+        // outer sub-steps of plane tid (Convection_2D.cpp:57-86)
+        double a = 0.0, b = 0.0;
+        for (int i = 0; i < kLineSlots; i++) {
+            a = fmax(a, redP[tid * kLineSlots + i]);
+            b = fmax(b, redR[tid * kLineSlots + i]);
+        }
+        const long qp = T.q0 + tid;
+        double nsP = 1, nsR = 1;
+        if (qp < NQ) {
+            if (NP >= 3) {
+                const double dP = g.Pg[((size_t)1 * NR + 0) * NQ + qp] - g.Pg[qp];
+                const double nd = (double)g.dt * a / dP / 0.25;
+                nsP = nd <= 1 ? 1 : ceil(nd);
+            }
+            if (NR >= 3) {
+                const double dR = g.Rg[((size_t)0 * NR + 1) * NQ + qp] - g.Rg[qp];
+                const double nd = (double)g.dt * b / dR / 0.25;
+                nsR = nd <= 1 ? 1 : ceil(nd);
+            }
+        }
+        const double ns = fast ? ((nsR < nsP) ? nsR : nsP)   // std::min
+                               : ((nsP < nsR) ? nsR : nsP);  // std::max
+        int nst = 0;
+        if (qp < NQ && ns <= kMaxSubsteps) nst = (int)ns;
+        else if (qp < NQ && ns > kMaxSubsteps) raise_error(status, VERB_ERR_COURANT);
+        ns_s[tid] = nst;
+        dtq_s[tid] = g.dt / ns;
+    }
+    __syncthreads();
+    int maxsteps = 0;
+    for (int i = 0; i < TQ; i++) maxsteps = max(maxsteps, ns_s[i]);
+    const int nsq = qok ? ns_s[tq] : 0;
+    const double dtq = dtq_s[tq];
+
+    // this lane's MLT line (r = li < NR-1) and R line (p = li): inner
+    // sub-steps, boundary data, geometry
+    const bool mline = qok && nsq > 0 && NP >= 3 && li < NR - 1;
+    const bool rline = qok && nsq > 0 && li < NP;
+    int ninP = 0, ninR = 0;
+    double dxP = 1.0, dxR = 1.0;
+    LineBC bcP{}, bcR{};
+    if (mline) {
+        dxP = g.Pg[((size_t)1 * NR + li) * NQ + q] - g.Pg[((size_t)0 * NR + li) * NQ + q];
+        ninP = inner_steps(dtq, umP, dxP, status);
+        bcP = line_bc(g, T, 0, 0, li, tq, g.G_conv);
+    }
+    if (rline && NR >= 3) {
+        dxR = g.Rg[((size_t)li * NR + 1) * NQ + q] - g.Rg[((size_t)li * NR + 0) * NQ + q];
+        ninR = inner_steps(dtq, umR, dxR, status);
+        bcR = line_bc(g, T, 1, li, 0, tq, g.G_conv);
+    }
+
+    for (int it = 0; it < maxsteps; it++) {
+        const bool act = it < nsq;
+        if (act && mline && ninP > 0) {
+            // MLT line r = li: points p at X[(p*NR + li)*TQ + tq]
+            double* xl = X + li * TQ + tq;
+            const int xs = NR * TQ;
+            for (int k = 0; k < ninP; k++) {
+                double C[NP];
+                line_courant<NP>(g.VP + (size_t)li * NQ + q, (int)pstride, dxP, dtq / ninP, C);
+                double xr[NP];
+                const double* xp = xl;
+                asm volatile("" : "+v"(xp));
+                sfor<0, NP>([&](auto ic) {
+                    constexpr int i = ic;
+                    xr[i] = xp[i * xs];
+                });
+                line_step<NP>(xr, C, bcP, xl, xs);
+            }
+        }
+        __syncthreads();
+        if (act && rline) {
+            // R line p = li: points r at X[(li*NR + r)*TQ + tq]
+            double* xl = X + li * NR * TQ + tq;
+            const size_t gb = (size_t)li * pstride + q;  // global index of (li, r = 0)
+            const double* gc = g.G_conv + gb;
+            asm volatile("" : "+v"(gc), "+v"(xl));
+            if (NR >= 3) {
+                // PSD_PR_G = PSD_PR.times(G_conv), R sweep, then / G_conv
+#pragma unroll 1
+                for (int i = 0; i < NR; i++) xl[i * TQ] = xl[i * TQ] * gc[(size_t)i * NQ];
+                if (ninR > 0) {
+                    for (int k = 0; k < ninR; k++) {
+                        double C[NR];
+                        line_courant<NR>(g.VR + gb, (int)NQ, dxR, dtq / ninR, C);
+                        double xr[NR];
+                        const double* xs = xl;
+                        asm volatile("" : "+v"(xs));
+                        sfor<0, NR>([&](auto ic) {
+                            constexpr int i = ic;
+                            xr[i] = xs[i * TQ];
+                        });
+                        line_step<NR>(xr, C, bcR, xl, TQ);
+                    }
+                }
+#pragma unroll 1
+                for (int i = 0; i < NR; i++) xl[i * TQ] = xl[i * TQ] / gc[(size_t)i * NQ];
+            }
+            // losses (exp(L*dt)) on every point, sources on interior R
+#pragma unroll 1
+            for (int i = 0; i < NR; i++) {
+                const size_t gi = gb + (size_t)i * NQ;
+                const double before = xl[i * TQ];
+                double v = before * exp(g.losses_conv[gi] * dtq);
+                if (lost) lost[gi] = lost[gi] + (before - v);
+                if (i >= 1 && i < NR - 1) v = v + g.sources[gi] * dtq;
+                xl[i * TQ] = v;
+            }
+        }
+        __syncthreads();
+    }
+    for (int t = tid; t < npt; t += nthreads) {
+        const int tq2 = t & (TQ - 1);
+        const int pr = t >> tqs;
+        const int p = pr / NR, r = pr - p * NR;
+        const long qq = T.q0 + tq2;
+        if (qq < NQ) psd[((size_t)p * NR + r) * NQ + qq] = X[t];
+    }
+}
+
+size_t convection_lines_lds_bytes(int nP, int nR, int TQ) {
+    return ((size_t)nP * nR * TQ + 2 * (size_t)TQ * kLineSlots + TQ) * sizeof(double) + TQ * sizeof(int);
+}
+
+// Line-kernel instantiations: (nP, nR) of the grids this build runs.
+template <int NP, int NR>
+int launch_lines_t(const verb_grid& g, double* psd, int* status, hipStream_t stream, int TQ, double* lost, int fast,
+                   int max_lds) {
+    const size_t lds = convection_lines_lds_bytes(NP, NR, TQ);
+    if (lds > (size_t)max_lds) return VERB_ERR_ARG;
+    hipFuncSetAttribute((const void*)convection_lines<NP, NR>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    const long NQ = (long)g.nV * g.nK;
+    const int ntiles = (int)((NQ + TQ - 1) / TQ);
+    const int per_xcd = (ntiles + 7) / 8;
+    hipLaunchKernelGGL((convection_lines<NP, NR>), dim3(per_xcd * 8), dim3(TQ * kLineSlots), lds, stream, g, psd,
+                       status, TQ, ntiles, per_xcd, lost, fast);
+    return hipGetLastError() == hipSuccess ? VERB_OK : VERB_ERR_HIP;
+}
+
+// Returns -1 when no instantiation covers (nP, nR).
+int launch_lines(const verb_grid& g, double* psd, int* status, hipStream_t stream, int TQ, double* lost, int fast,
+                 int max_lds) {
+    if (g.nP == 24 && g.nR == 29) return launch_lines_t<24, 29>(g, psd, status, stream, TQ, lost, fast, max_lds);
+    if (g.nP == 24 && g.nR == 10) return launch_lines_t<24, 10>(g, psd, status, stream, TQ, lost, fast, max_lds);
+    return -1;
+}
+
 }  // namespace
 
 size_t convection_lds_bytes(int nP, int nR, int TQ) {
@@ -636,7 +1029,7 @@ size_t convection_lds_bytes(int nP, int nR, int TQ) {
 int convection_tq = 0;  // 0 = choose automatically
 
 int launch_convection_mode(const verb_grid& g, double* psd, int* status, hipStream_t stream, int single_outer,
-                           int tq, double* lost, int fast) {
+                           int tq, double* lost, int fast, int kernel) {
     if (g.nP < 1 || g.nR < 1) return VERB_ERR_ARG;
     if ((g.nP >= 3 && g.nP < 6) || (g.nR >= 3 && g.nR < 6)) return VERB_ERR_ARG;  // stencil needs 6 points
     if ((g.Pl_type == VERB_BC_PERIODIC) != (g.Pu_type == VERB_BC_PERIODIC)) return VERB_ERR_ARG;
@@ -647,6 +1040,14 @@ int launch_convection_mode(const verb_grid& g, double* psd, int* status, hipStre
     hipGetDevice(&dev);
     int max_lds = 0;
     hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    if (kernel == 0 && !single_outer) {
+        // the line kernel where it is instantiated for (nP, nR): TQ planes
+        // per workgroup, 32 lanes each
+        const int tql = tq > 0 ? tq : 8;
+        if ((tql & (tql - 1)) || tql * kLineSlots > 512) return VERB_ERR_ARG;  // (launch bounds)
+        const int rc = launch_lines(g, psd, status, stream, tql, lost, fast, max_lds);
+        if (rc >= 0) return rc;
+    }
     int TQ = tq > 0 ? tq : convection_tq;
     if (TQ <= 0) {
         TQ = 4;
@@ -665,8 +1066,8 @@ int launch_convection_mode(const verb_grid& g, double* psd, int* status, hipStre
 }
 
 int launch_convection(const verb_grid& g, double* psd, int* status, hipStream_t stream, int tq, double* lost,
-                      int fast) {
-    return launch_convection_mode(g, psd, status, stream, 0, tq, lost, fast);
+                      int fast, int kernel) {
+    return launch_convection_mode(g, psd, status, stream, 0, tq, lost, fast, kernel);
 }
 
 }  // namespace verb

@@ -127,7 +127,8 @@ class Tuning(ctypes.Structure):
     """ctypes mirror of ``verb_tuning``."""
 
     _fields_ = (("convection_tq", ctypes.c_int), ("lu_rb", ctypes.c_int), ("lu_cb", ctypes.c_int),
-                ("lu_persistent", ctypes.c_int), ("convection_ms", ctypes.c_double), ("local_ms", ctypes.c_double))
+                ("lu_persistent", ctypes.c_int), ("convection_ms", ctypes.c_double), ("local_ms", ctypes.c_double),
+                ("convection_kernel", ctypes.c_int))
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -243,8 +243,8 @@ class DeviceGrid:
                                            ctypes.byref(t)))
         return t.as_dict()
 
-    def set_tuning(self, convection_tq=0, lu_rb=0, lu_cb=0, lu_persistent=0):
-        t = Tuning(convection_tq, lu_rb, lu_cb, lu_persistent, 0.0, 0.0)
+    def set_tuning(self, convection_tq=0, lu_rb=0, lu_cb=0, lu_persistent=0, convection_kernel=0):
+        t = Tuning(convection_tq, lu_rb, lu_cb, lu_persistent, 0.0, 0.0, convection_kernel)
         self.ctx.check(lib().verb_set_tuning(self.ctx.handle, ctypes.byref(t)))
 
     def step_graph(self, k0=None):
