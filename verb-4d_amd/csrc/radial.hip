@@ -44,10 +44,21 @@ __global__ __launch_bounds__(256) void radial_kernel(const verb_grid g, double* 
 
     double bet = 0.0, aprev_c = 0.0;  // c[j-1] carried into row j
     bool bc_ok = true;
+    // Per-point inputs loaded without a branch, from a clamped row (the
+    // values of rows past the line are never used): the loads of later rows
+    // are free to be issued ahead of the recurrence, instead of each row
+    // waiting for its own.
+    double ps[MAXR], sr[MAXR], lr[MAXR];
+#pragma unroll
+    for (int j = 0; j < MAXR; j++) {
+        const size_t pj = base + (size_t)min(j, n - 1) * VK;
+        ps[j] = psd[pj];
+        sr[j] = g.sources[pj];
+        lr[j] = g.losses_radial[pj];
+    }
 #pragma unroll
     for (int j = 0; j < MAXR; j++) {
         if (j < n) {
-            const size_t pj = base + (size_t)j * VK;
             double a = 0.0, b = 0.0, c = 0.0, B0 = 0.0, C0 = 0.0;
             if (j == 0) {
                 C0 = g.Ll[bidx];
@@ -64,8 +75,8 @@ __global__ __launch_bounds__(256) void radial_kernel(const verb_grid g, double* 
             } else {
                 b += 1.0 / dt;
                 B0 += 1.0 / dt;
-                C0 += g.sources[pj] * rs;
-                b -= g.losses_radial[pj];
+                C0 += sr[j] * rs;
+                b -= lr[j];
                 {   // ("x_left", "x_right"): dx1 = -1, dx2 = +1
                     const double dh1 = (x0) - (xm);
                     const double dh21 = (x0) - (xp);
@@ -88,7 +99,7 @@ __global__ __launch_bounds__(256) void radial_kernel(const verb_grid g, double* 
                 }
             }
             double r = C0;
-            r += B0 * psd[pj];
+            r += B0 * ps[j];
             // Thomas forward sweep (tridag)
             if (j == 0) {
                 u[0] = r / (bet = b);
@@ -102,8 +113,8 @@ __global__ __launch_bounds__(256) void radial_kernel(const verb_grid g, double* 
             xm = x0; x0 = xp;
             Dm = D0; D0 = Dp;
             Gm = G0; G0 = Gp;
-            if (j + 2 < n) {
-                const size_t pn = base + (size_t)(j + 2) * VK;
+            {
+                const size_t pn = base + (size_t)min(j + 2, n - 1) * VK;
                 xp = g.Lg[pn];
                 Dp = g.DLL[pn];
                 Gp = g.G_radial[pn];
