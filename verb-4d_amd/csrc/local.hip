@@ -182,15 +182,20 @@ constexpr int kAsmTile = 16;
 __global__ __launch_bounds__(kAsmTile* kAsmTile) void local_assemble_tiled(
     const verb_grid g, const double* __restrict__ psd, double* __restrict__ b0, double* __restrict__ c0,
     double* __restrict__ rhs, double* __restrict__ diag, double dt_sub, int premul, int* status,
-    const int* __restrict__ k0s) {
-    const int node = blockIdx.y;
+    const int* __restrict__ k0s, int tiles, int per_xcd) {
+    // XCD-aware order: consecutive workgroup ids go to different XCDs in
+    // turn, so each XCD takes a contiguous run of (node, tile) pairs and the
+    // stencil halos of neighbouring tiles of a node meet in one L2
+    const int lin = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+    if (lin >= tiles * g.nP * g.nR) return;
+    const int node = lin / tiles, tile = lin - node * tiles;
     const int iP = node / g.nR, iR = node % g.nR;
     if (node_skipped(g, iR)) return;
     const int k0 = k0s ? k0s[node] : 0;
     const int nx = g.nV, ny = g.nK - k0, n = nx * ny;
     const int nfull = g.nV * g.nK;
     const int tiles_y = (g.nK + kAsmTile - 1) / kAsmTile;
-    const int ix0 = (blockIdx.x / tiles_y) * kAsmTile, iy0 = (blockIdx.x % tiles_y) * kAsmTile;
+    const int ix0 = (tile / tiles_y) * kAsmTile, iy0 = (tile % tiles_y) * kAsmTile;
     if (ix0 >= nx || iy0 >= ny) return;
     const int tx = threadIdx.x % kAsmTile, ty = threadIdx.x / kAsmTile;
     __shared__ double out[12][kAsmTile][kAsmTile + 1];
@@ -221,8 +226,8 @@ __global__ __launch_bounds__(kAsmTile* kAsmTile) void local_assemble_tiled(
     const int ix = ix0 + tx, iy = iy0 + ty;
     if (ix < nx && iy < ny) {
         const int in = iy * nx + ix;
-        b0[nn + in] = out[9][tx][ty];
-        c0[nn + in] = out[10][tx][ty];
+        if (b0) b0[nn + in] = out[9][tx][ty];
+        if (c0) c0[nn + in] = out[10][tx][ty];
         rhs[nn + in] = out[11][tx][ty];
         if (diag) {
             double* dn = diag + (size_t)node * 9 * nfull;
@@ -242,8 +247,9 @@ void launch_assemble(const verb_grid& g, const double* psd, double* ab, double* 
                            stream, g, psd, ab, b0, c0, rhs, diag, dt_sub, write_band, premul, status, k0s);
     } else {
         const int tiles = ((g.nV + kAsmTile - 1) / kAsmTile) * ((g.nK + kAsmTile - 1) / kAsmTile);
-        hipLaunchKernelGGL(local_assemble_tiled, dim3(tiles, nodes), dim3(kAsmTile * kAsmTile), 0, stream, g, psd,
-                           b0, c0, rhs, diag, dt_sub, premul, status, k0s);
+        const int per_xcd = (tiles * nodes + 7) / 8;
+        hipLaunchKernelGGL(local_assemble_tiled, dim3(8 * per_xcd), dim3(kAsmTile * kAsmTile), 0, stream, g, psd,
+                           b0, c0, rhs, diag, dt_sub, premul, status, k0s, tiles, per_xcd);
     }
 }
 
@@ -592,7 +598,8 @@ int launch_local(const verb_grid& g, double* psd, const LocalWork& w, bool facto
     if (window && factor && num_substeps == 1 && !cached) {
         // plain path: factor with the forward elimination fused, then U x = y
         if (marks) hipEventRecord(marks[0], stream);
-        launch_assemble(g, psd, w.ab, w.b0, w.c0, w.rhs, w.diag, dt_sub, 0, 1, status, k0s, stream);
+        // (B0 and C0 are read only by the sub-stepped and cached solves)
+        launch_assemble(g, psd, w.ab, nullptr, nullptr, w.rhs, w.diag, dt_sub, 0, 1, status, k0s, stream);
         if (marks) {
             hipEventRecord(marks[1], stream);
             hipEventRecord(marks[2], stream);
