@@ -65,10 +65,12 @@ def lu_flops(n, kl, ku):
     return float(np.sum(l + 2 * l * u))
 
 
-def phase_work(w, fused):
+def phase_work(w, fused, to_psd=False):
     """Algorithmic work per launch of each kernel phase: (amount, unit, bound).
     fused: the back-substitution runs inside the persistent LU kernel (the
-    local_factor phase), and the local_solve phase is the PSD scatter only."""
+    local_factor phase), and the local_solve phase is the PSD scatter only;
+    to_psd: that back-substitution also writes x straight into the PSD, so
+    the local_solve phase launches nothing (None)."""
     n = w.nV * w.nK
     kl = ku = w.nV + 1
     nodes = w.nP * w.nR
@@ -81,7 +83,7 @@ def phase_work(w, fused):
             # LU flops; the same launch also streams U back for x = U^-1 y
             "local_factor": (lu_flops(n, kl, ku) * nodes, "flop", "mfma"),
             # x read, PSD written
-            "local_solve": (16.0 * pts, "B", "hbm"),
+            "local_solve": None if to_psd else (16.0 * pts, "B", "hbm"),
         }
     return {
         # PSD read+write, VP, VR, G_conv, losses, sources: 7 doubles / point
@@ -501,11 +503,14 @@ def main():
     # the autotuner's persistent LU queue runs the back-substitution in the
     # same launch
     fused = bool((tuning or {}).get("lu_persistent", 0))
-    work = phase_work(w, fused)
+    # register windows of 4 or more wavefronts hold the back-substitution,
+    # which then writes x into the PSD itself
+    to_psd = fused and (tuning or {}).get("lu_rb", 0) >= 4
+    work = phase_work(w, fused, to_psd)
     phase_out = {}
     dom, dom_ms = None, -1.0
     for p, (ms, cnt) in phases.items():
-        if cnt == 0:
+        if cnt == 0 or work[p] is None:
             continue
         avg = ms / cnt
         amount, unit, bound = work[p]
@@ -524,8 +529,8 @@ def main():
                 "frac": d["frac"], "traffic": profiled_traffic(dom), "kernel": dom}
     if fused and dom == "local_factor":
         roofline["note"] = ("band_lu_window with the back-substitution fused in (the persistent launch solves "
-                            "x = U^-1 y on the workgroups that run out of nodes to factor); achieved = LU flops / "
-                            "launch time")
+                            "x = U^-1 y on the workgroups that run out of nodes to factor%s); achieved = LU flops / "
+                            "launch time" % (", writing x into the PSD" if to_psd else ""))
 
     cached = None
     if not args.no_cached:

@@ -34,7 +34,19 @@ struct LuArgsT {
     int* bs_flags = nullptr;
     int* bs_queue = nullptr;
     T* bs_x = nullptr;
+    // ... and, when bs_psd is given, x goes straight into the node's PSD
+    // slice instead of bs_x: [P][R][V][K] with nK pitch-angle points per
+    // energy, the node's unknowns starting at pitch-angle index bs_k0[node]
+    // (null: 0), row r = iy*nx + ix at ix*bs_nK + k0 + iy
+    T* bs_psd = nullptr;
+    const int* bs_k0 = nullptr;
+    int bs_nK = 0;
 };
+
+// Whether the register-window LU that launch_band_lu picks for kl
+// (variant < 0: the smallest that covers it) runs the back-substitution in
+// its persistent launch (it needs at least 256 threads per workgroup).
+bool band_lu_fuses_backsolve(int variant, int kl);
 
 // Blocked U geometry for n unknowns and bandwidths kl, ku: columns per block
 // (kl+ku+16 rounded up to 16) and the per-node stride. Returns false when the

@@ -300,10 +300,19 @@ constexpr int kBsMaxSpins = 1 << 18;
 // U blocks in flight per fused back-substitution: one node per workgroup, so
 // its latency is hidden by prefetch depth (the LU's register budget allows it)
 constexpr int kBsFusedSets = 3;
+// Where a back-substitution stores x besides its LDS ring: x_all in
+// energy-fast order, or, with psd given, the node's PSD slice (row r = iy*nx
+// + ix at psd[ix*nK + k0 + iy]: the scatter back to [V][K])
+template <typename T>
+struct XOut {
+    T* psd = nullptr;
+    int nx = 1, nK = 0, k0 = 0;
+};
 template <typename T, bool BLK, int NI, int D>
 __device__ __forceinline__ void backsolve_node(const T* __restrict__ ab_all, const T* __restrict__ y_all,
                                                T* __restrict__ x_all, int n, int stride, int kl, int ku, int ldab,
-                                               int node, int ublk_cols, size_t ublk_stride, T* sh, int tid);
+                                               int node, int ublk_cols, size_t ublk_stride, T* sh, int tid,
+                                               XOut<T> xout = XOut<T>());
 
 // (FP32 windows up to 8x14 fit 128 registers: two workgroups per CU)
 template <typename T, int RB, int CB>
@@ -1153,8 +1162,15 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
                 __syncthreads();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                 if (bs_skip) continue;
+                XOut<T> xout;
+                if (a.bs_psd) {
+                    xout.psd = a.bs_psd + (size_t)node * a.n;
+                    xout.nx = nx;
+                    xout.nK = a.bs_nK;
+                    xout.k0 = a.bs_k0 ? a.bs_k0[node] : 0;
+                }
                 backsolve_node<T, true, CB, kBsFusedSets>(a.ab, a.y, a.bs_x, a.nn ? a.nn[node] : a.n, a.n, kl, kl, ldab, node,
-                                               a.ublk_cols, a.ublk_stride, sh, tid);
+                                               a.ublk_cols, a.ublk_stride, sh, tid, xout);
             }
         }
     }
@@ -1180,7 +1196,8 @@ constexpr int kBsMaxI = 20;  // staged 16-column groups per thread: (kl+ku+16)/1
 template <typename T, bool BLK, int NI, int D>
 __device__ __forceinline__ void backsolve_node(const T* __restrict__ ab_all, const T* __restrict__ y_all,
                                                T* __restrict__ x_all, int n, int stride, int kl, int ku, int ldab,
-                                               int node, int ublk_cols, size_t ublk_stride, T* sh, int tid) {
+                                               int node, int ublk_cols, size_t ublk_stride, T* sh, int tid,
+                                               XOut<T> xout) {
     const bool act = tid < kBsThreads;
     const int kv = kl + ku;
     const int WS = BLK ? ublk_cols : kv + NB;  // columns per block (relative to the block start)
@@ -1277,7 +1294,14 @@ __device__ __forceinline__ void backsolve_node(const T* __restrict__ ab_all, con
             const T xi = b * rd;
             if (tid < NB) {
                 x[(jb + i) & (kXRing - 1)] = xi;
-                if (r < n) xo[r] = xi;
+                if (r < n) {
+                    if (xout.psd) {
+                        const int iy = r / xout.nx, ix = r - iy * xout.nx;
+                        xout.psd[(size_t)ix * xout.nK + xout.k0 + iy] = xi;
+                    } else {
+                        xo[r] = xi;
+                    }
+                }
             }
             __builtin_amdgcn_s_setprio(0);
         }
@@ -1562,7 +1586,9 @@ int launch_band_lu_t(const LuArgsT<T>& a, int nodes, hipStream_t stream) {
         grid = min(nodes, max(1, cus));
         if (a.bs_queue && RB * 64 < kBsThreads) {
             // too few threads for the back-substitution in the same kernel:
-            // the LU, then the back-substitution kernel
+            // the LU, then the back-substitution kernel (which writes x in
+            // energy-fast order only)
+            if (a.bs_psd) return VERB_ERR_ARG;
             LuArgsT<T> b = a;
             b.bs_queue = nullptr;
             b.bs_flags = nullptr;
@@ -1576,8 +1602,8 @@ int launch_band_lu_t(const LuArgsT<T>& a, int nodes, hipStream_t stream) {
             return VERB_ERR_ARG;
         }
         if (a.bs_queue) {
-            if (sizeof(T) != 8 || !a.bs_flags || !a.bs_x || !a.y || a.write_L || a.ublk_cols <= 0 ||
-                a.ublk_cols > CB * NB)
+            if (sizeof(T) != 8 || !a.bs_flags || (!a.bs_x && !a.bs_psd) || !a.y || a.write_L || a.ublk_cols <= 0 ||
+                a.ublk_cols > CB * NB || (a.bs_psd && a.bs_nK < 1))
                 return VERB_ERR_ARG;
             hipLaunchKernelGGL(zero_lu_counters, dim3((nodes + 255) / 256), dim3(256), 0, stream, a.queue, a.bs_queue,
                                a.bs_flags, nodes);
@@ -1642,6 +1668,13 @@ int launch_band_lu(const LuArgs& a, int nodes, hipStream_t stream, int variant) 
     for (const auto& v : kLuVariants)
         if (lu_variant_fits(v, kl)) return v.launch(a, nodes, stream);
     return VERB_ERR_ARG;
+}
+
+bool band_lu_fuses_backsolve(int variant, int kl) {
+    if (variant >= 0) return variant < lu_variant_count() && kLuVariants[variant].rb * 64 >= kBsThreads;
+    for (const auto& v : kLuVariants)
+        if (lu_variant_fits(v, kl)) return v.rb * 64 >= kBsThreads;
+    return false;
 }
 
 int launch_band_lu32(const LuArgsT<float>& a, int nodes, hipStream_t stream) {

@@ -616,10 +616,18 @@ int launch_local(const verb_grid& g, double* psd, const LocalWork& w, bool facto
         // persistent launch with blocked U: the back-substitution runs in the
         // same kernel, on the workgroups that run out of nodes to factor
         const bool fused = w.queue && la.ublk_cols > 0;
+        // ... and, when that launch is wide enough to hold it, writes x
+        // straight back into the PSD (no scatter pass)
+        const bool to_psd = fused && band_lu_fuses_backsolve(w.lu_variant, kl);
         if (fused) {
             la.bs_queue = w.queue + 1;
             la.bs_flags = w.queue + 2;
             la.bs_x = w.rhs;
+        }
+        if (to_psd) {
+            la.bs_psd = psd;
+            la.bs_k0 = k0s;
+            la.bs_nK = g.nK;
         }
         int rc = launch_band_lu(la, nodes, stream, w.lu_variant);
         if (rc) return rc;
@@ -632,7 +640,7 @@ int launch_local(const verb_grid& g, double* psd, const LocalWork& w, bool facto
                                   nns, order, nullptr, la.ublk_cols, la.ublk_stride);
             if (rc) return rc;
         }
-        hipLaunchKernelGGL(scatter_psd, agrid, dim3(kAsmThreads), 0, stream, g, w.rhs, psd, k0s);
+        if (!to_psd) hipLaunchKernelGGL(scatter_psd, agrid, dim3(kAsmThreads), 0, stream, g, w.rhs, psd, k0s);
         if (marks) hipEventRecord(marks[5], stream);
         return hipGetLastError() == hipSuccess ? VERB_OK : VERB_ERR_HIP;
     }
