@@ -261,7 +261,7 @@ struct Lds {
     T pnl[2][NB * PS];                       // panels p and p+1, column-major (full-row interchanges)
     T u12[NB * W];
     T sb[2 * NB * W];                        // pre-interchange contents of the rows that move
-    T stash[CB * 4 * 64];                    // wavefront 0's accumulators while it factors
+    T stash[(CB - 1) * 4 * 64];              // slots 1..CB-1 of the factoring wavefront's tiles
     T rw[PR];                                // right-hand-side window (index mod PR)
     T linv[NB * LS];                         // L11^-1 of the current panel, row-major
     int slot[PR];                            // sb row holding window row i's old contents, -1: none
@@ -272,7 +272,16 @@ struct Lds {
     unsigned jz;                             // columns of the current panel with a zero pivot (bit c)
     int lfin;                                // the panel's multipliers are all finite
     int qnode;                               // persistent mode: current dispatch slot
+    // The wavefront that shares a SIMD with the factoring one hands the
+    // trailing update of NV of its tiles to wavefronts on other SIMDs (an
+    // FP64 MFMA holds the SIMD's FP64 pipeline for its 64 cycles, during
+    // which the factorisation issues nothing): the tiles go out and come back
+    // through vx ([tile][lane][4]), vflag = the panel whose tiles are there
+    static constexpr int NV = (sizeof(T) == 8 && RB == 8 && CB == 14) ? 8 : 0;
+    T vx[NV > 0 ? NV * 256 : 1];
+    int vflag;
 };
+static_assert(sizeof(Lds<double, 8, 14>) <= 160 * 1024, "LDS of the 8x14 window");
 
 }  // namespace
 
@@ -368,6 +377,7 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
     if (tid == 0) {
         S.ju = 0;
         S.info = 0;
+        S.vflag = -1;
     }
     __syncthreads();
 
@@ -387,7 +397,7 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
 #pragma unroll
             for (int s = 1; s < CB; s++)
 #pragma unroll
-                for (int r = 0; r < 4; r++) S.stash[(s * 4 + r) * 64 + l] = acc[s][r];
+                for (int r = 0; r < 4; r++) S.stash[((s - 1) * 4 + r) * 64 + l] = acc[s][r];
     }
     __syncthreads();
     int ju_prev = 0;
@@ -399,6 +409,8 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
         const int nc = min(NB, n - j0);
         T* const pn = S.pnl[p & 1];
         const bool fac = (w == (p + RB - 1) % RB);
+        constexpr int NV = L::NV;
+        bool vpartner = false;  // this wavefront's tiles 2..NV+1 are updated by others in this stage
 
         // 2. factorisation of panel p in registers. Lane l keeps
         //    window rows l + 64q (q < NQ) as 16-wide vectors; an interchange
@@ -722,10 +734,26 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
                     for (int s = 2; s < CB; s++) acc[s][r] = S.sb[dfx[r] * W + NB * (s - 1) + jj];
                 }
             }
-            // slot 1 was updated in step 4a
+            // slot 1 was updated in step 4a. The wavefront on the factoring
+            // one's SIMD (local row block 4 of 8: wavefront w runs on SIMD
+            // w mod 4) hands tiles 2..NV+1 to the wavefronts of row blocks
+            // 2, 3, 5, 6 and 7, which take them after their own tiles.
+            if constexpr (NV > 0) {
+                vpartner = lb == 4;
+                if (vpartner) {
+                    static_for<0, NV>([&](auto tc) {
+                        constexpr int t = tc;
+                        if (NB * (q + 2 + t) > ju_prev) return;
+#pragma unroll
+                        for (int r = 0; r < 4; r++) S.vx[(t * 64 + l) * 4 + r] = acc[2 + t][r];
+                    });
+                    if (l == 0) __hip_atomic_store(&S.vflag, p, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+            }
 #pragma unroll
             for (int s = 2; s < CB; s++) {
                 if (NB * (q + s) > ju_prev) continue;
+                if (NV > 0 && vpartner && s < 2 + NV) continue;
                 A4 c4 = acc[s];
 #pragma unroll
                 for (int kk = 0; kk < 4; kk++) {
@@ -746,8 +774,42 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
                 const int sft = 1 + (lb - 1) + t * (RB - 1);
                 if (p < npan && sft < CB)
 #pragma unroll
-                    for (int r = 0; r < 4; r++) S.stash[(sft * 4 + r) * 64 + l] = ft[t][r];
+                    for (int r = 0; r < 4; r++) S.stash[((sft - 1) * 4 + r) * 64 + l] = ft[t][r];
             });
+            if constexpr (NV > 0) {
+                // the row-block-4 tiles this wavefront takes (t = h, h + 5, ... for
+                // row blocks 2, 3, 5, 6, 7), with the same MFMA sequence, so
+                // the same values, as their owner's
+                const int h = (lb == 2) ? 0 : (lb == 3) ? 1 : (lb == 5) ? 2 : (lb == 6) ? 3 : (lb == 7) ? 4 : -1;
+                if (h >= 0) {
+                    // (the owner publishes its tiles at the start of its
+                    // trailing update; bounded wait)
+                    int spins = 0;
+                    while (__hip_atomic_load(&S.vflag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != p &&
+                           spins < (1 << 16)) {
+                        __builtin_amdgcn_s_sleep(1);
+                        spins++;
+                    }
+                    if (spins >= (1 << 16) && l == 0) raise_error(a.status, VERB_ERR_HIP);
+                    T ap[4];
+#pragma unroll
+                    for (int kk = 0; kk < 4; kk++) ap[kk] = -pn[(4 * kk + g) * PS + NB * 4 + jj];
+#pragma unroll 1
+                    for (int t = h; t < NV; t += 5) {
+                        if (NB * (q + 2 + t) > ju_prev) continue;
+                        A4 c4;
+#pragma unroll
+                        for (int r = 0; r < 4; r++) c4[r] = S.vx[(t * 64 + l) * 4 + r];
+#pragma unroll
+                        for (int kk = 0; kk < 4; kk++) {
+                            const T bv = S.u12[(4 * kk + g) * W + NB * (1 + t) + jj];
+                            c4 = P::mfma(ap[kk], bv, c4);
+                        }
+#pragma unroll
+                        for (int r = 0; r < 4; r++) S.vx[(t * 64 + l) * 4 + r] = c4[r];
+                    }
+                }
+            }
             // Right-hand-side window of panel q, on the wavefront that holds
             // the next panel rows (it has the least trailing-update work):
             // the panel's interchanges, then per column c the multipliers
@@ -838,6 +900,18 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
             }
         }
         __syncthreads();
+        if constexpr (NV > 0) {
+            // tiles 2..NV+1 of the stage, now in slots 1..NV
+            if (vpartner) {
+                const int q = p - 1, l = lane_fresh();
+                static_for<0, NV>([&](auto tc) {
+                    constexpr int t = tc;
+                    if (NB * (q + 2 + t) > ju_prev) return;
+#pragma unroll
+                    for (int r = 0; r < 4; r++) acc[1 + t][r] = S.vx[(t * 64 + l) * 4 + r];
+                });
+            }
+        }
         if (p == npan) break;
         const int lb = (w - p % RB + RB) % RB;  // local row block of this wavefront
         const bool lfin = S.lfin != 0;
@@ -913,7 +987,7 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
 #pragma unroll
             for (int s = 1; s < CB; s++)
 #pragma unroll
-                for (int r = 0; r < 4; r++) acc[s][r] = S.stash[(s * 4 + r) * 64 + l];
+                for (int r = 0; r < 4; r++) acc[s][r] = S.stash[((s - 1) * 4 + r) * 64 + l];
         }
 
         // the panel's ipiv and U fill extents (dgbtf2: ju = max(ju, min(j +
