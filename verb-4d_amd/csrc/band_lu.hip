@@ -276,10 +276,10 @@ struct Lds {
     // trailing update of NV of its tiles to wavefronts on other SIMDs (an
     // FP64 MFMA holds the SIMD's FP64 pipeline for its 64 cycles, during
     // which the factorisation issues nothing): the tiles go out and come back
-    // through vx ([tile][lane][4]), vflag = the panel whose tiles are there
+    // through vx ([tile][4][lane]), vflag = the panel whose tiles are there
     static constexpr int NV = (sizeof(T) == 8 && RB == 8 && CB == 14) ? 8 : 0;
     T vx[NV > 0 ? NV * 256 : 1];
-    int vflag;
+    int vflag, vdone;
 };
 static_assert(sizeof(Lds<double, 8, 14>) <= 160 * 1024, "LDS of the 8x14 window");
 
@@ -378,6 +378,7 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
         S.ju = 0;
         S.info = 0;
         S.vflag = -1;
+        S.vdone = 0;
     }
     __syncthreads();
 
@@ -745,9 +746,12 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
                         constexpr int t = tc;
                         if (NB * (q + 2 + t) > ju_prev) return;
 #pragma unroll
-                        for (int r = 0; r < 4; r++) S.vx[(t * 64 + l) * 4 + r] = acc[2 + t][r];
+                        for (int r = 0; r < 4; r++) S.vx[(t * 4 + r) * 64 + l] = acc[2 + t][r];
                     });
-                    if (l == 0) __hip_atomic_store(&S.vflag, p, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    if (l == 0) {
+                        S.vdone = 0;
+                        __hip_atomic_store(&S.vflag, p, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    }
                 }
             }
 #pragma unroll
@@ -799,15 +803,16 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
                         if (NB * (q + 2 + t) > ju_prev) continue;
                         A4 c4;
 #pragma unroll
-                        for (int r = 0; r < 4; r++) c4[r] = S.vx[(t * 64 + l) * 4 + r];
+                        for (int r = 0; r < 4; r++) c4[r] = S.vx[(t * 4 + r) * 64 + l];
 #pragma unroll
                         for (int kk = 0; kk < 4; kk++) {
                             const T bv = S.u12[(4 * kk + g) * W + NB * (1 + t) + jj];
                             c4 = P::mfma(ap[kk], bv, c4);
                         }
 #pragma unroll
-                        for (int r = 0; r < 4; r++) S.vx[(t * 64 + l) * 4 + r] = c4[r];
+                        for (int r = 0; r < 4; r++) S.vx[(t * 4 + r) * 64 + l] = c4[r];
                     }
+                    if (l == 0) __hip_atomic_fetch_add(&S.vdone, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
                 }
             }
             // Right-hand-side window of panel q, on the wavefront that holds
@@ -898,20 +903,27 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
                     }
                 }
             }
-        }
-        __syncthreads();
-        if constexpr (NV > 0) {
-            // tiles 2..NV+1 of the stage, now in slots 1..NV
-            if (vpartner) {
-                const int q = p - 1, l = lane_fresh();
-                static_for<0, NV>([&](auto tc) {
-                    constexpr int t = tc;
-                    if (NB * (q + 2 + t) > ju_prev) return;
+            if constexpr (NV > 0) {
+                // tiles 2..NV+1, now in slots 1..NV, back once the five
+                // helpers are done (bounded wait)
+                if (vpartner) {
+                    int spins = 0;
+                    while (__hip_atomic_load(&S.vdone, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != 5 &&
+                           spins < (1 << 16)) {
+                        __builtin_amdgcn_s_sleep(1);
+                        spins++;
+                    }
+                    if (spins >= (1 << 16) && l == 0) raise_error(a.status, VERB_ERR_HIP);
+                    static_for<0, NV>([&](auto tc) {
+                        constexpr int t = tc;
+                        if (NB * (q + 2 + t) > ju_prev) return;
 #pragma unroll
-                    for (int r = 0; r < 4; r++) acc[1 + t][r] = S.vx[(t * 64 + l) * 4 + r];
-                });
+                        for (int r = 0; r < 4; r++) acc[1 + t][r] = S.vx[(t * 4 + r) * 64 + l];
+                    });
+                }
             }
         }
+        __syncthreads();
         if (p == npan) break;
         const int lb = (w - p % RB + RB) % RB;  // local row block of this wavefront
         const bool lfin = S.lfin != 0;
