@@ -280,6 +280,7 @@ struct Lds {
     static constexpr int NV = (sizeof(T) == 8 && RB == 8 && CB == 14) ? 8 : 0;
     T vx[NV > 0 ? NV * 256 : 1];
     int vflag, vdone;
+    int u12done;                             // wavefronts done with their deferred U12 tiles in this stage
 };
 static_assert(sizeof(Lds<double, 8, 14>) <= 160 * 1024, "LDS of the 8x14 window");
 
@@ -379,6 +380,7 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
         S.info = 0;
         S.vflag = -1;
         S.vdone = 0;
+        S.u12done = 0;
     }
     __syncthreads();
 
@@ -405,6 +407,11 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
     // sb rows whose contents this wavefront's rows r take in slots 2..CB-1,
     // applied in the next stage's trailing update (-1: none)
     int dfx[4] = {-1, -1, -1, -1};
+    // U12 of the previous panel beyond its first column block, formed at the
+    // start of the next stage by the wavefronts of row blocks 1..3 and 5..7
+    // (finite multipliers, L not kept): the sb rows its moved rows took
+    int dslc[4] = {-1, -1, -1, -1};
+    bool u12_later = false;
     for (int p = 0;; p++) {
         const int j0 = NB * p;
         const int nc = min(NB, n - j0);
@@ -726,6 +733,31 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
             T an[4];
 #pragma unroll
             for (int kk = 0; kk < 4; kk++) an[kk] = -pn[(4 * kk + g) * PS + rowbase + jj];
+            // U12 tiles 1..CB-2 of panel q (tile 0 was formed in step 3b for
+            // step 4a), dealt out to the wavefronts of row blocks 1..3 and 5..7 (not on
+            // the factoring wavefront's SIMD), as step 3b forms them
+            if (u12_later) {
+                const int hh = (lb < 4) ? lb - 1 : lb - 2;  // 0..5 (row block 4: 3, unused)
+                if (lb != 4) {
+                    T la[4];
+#pragma unroll
+                    for (int kk = 0; kk < 4; kk++) la[kk] = S.linv[jj * L::LS + 4 * kk + g];
+                    static_for<0, (CB - 2 + 5) / 6>([&](auto tc) {
+                        constexpr int t = tc;
+                        const int cb = 1 + hh + 6 * t;
+                        if (cb > CB - 2) return;
+                        A4 d = A4{T(0), T(0), T(0), T(0)};
+#pragma unroll
+                        for (int kk = 0; kk < 4; kk++) {
+                            const T bv = (dslc[kk] >= 0) ? S.sb[dslc[kk] * W + NB * cb + jj] : T(0);
+                            d = P::mfma(la[kk], bv, d);
+                        }
+#pragma unroll
+                        for (int r = 0; r < 4; r++) S.u12[P::crow(g, r) * W + NB * cb + jj] = d[r];
+                    });
+                    if (l == 0) __hip_atomic_fetch_add(&S.u12done, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+            }
             // the contents that rows moved by panel q's interchanges take in
             // slots 2..CB-1 (slot 1 took them in step 3c)
 #pragma unroll
@@ -734,6 +766,16 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
 #pragma unroll
                     for (int s = 2; s < CB; s++) acc[s][r] = S.sb[dfx[r] * W + NB * (s - 1) + jj];
                 }
+            }
+            if (u12_later) {
+                // (bounded wait for the six)
+                int spins = 0;
+                while (__hip_atomic_load(&S.u12done, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != RB - 2 &&
+                       spins < (1 << 16)) {
+                    __builtin_amdgcn_s_sleep(1);
+                    spins++;
+                }
+                if (spins >= (1 << 16) && l == 0) raise_error(a.status, VERB_ERR_HIP);
             }
             // slot 1 was updated in step 4a. The wavefront on the factoring
             // one's SIMD (local row block 4 of 8: wavefront w runs on SIMD
@@ -1057,7 +1099,26 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
         //     rows: with finite multipliers as L11^-1 times the moved rows,
         //     one 16x16 tile per MFMA chain (column blocks dealt out to the
         //     wavefronts); otherwise by substitution, one lane per column
-        if (lb != 0 && lfin) {
+        // (with RB = 8 and L not kept, only tile 0, for step 4a; the others
+        // at the start of the next stage)
+        u12_later = RB == 8 && lfin && !a.write_L && p + 1 <= npan;
+        if (tid == 0) S.u12done = 0;  // (nobody waits on it until the next stage)
+        if (lb != 0 && lfin && u12_later) {
+            const int l = lane_fresh(), g = l >> 4, jj = l & 15;
+#pragma unroll
+            for (int kk = 0; kk < 4; kk++) dslc[kk] = (4 * kk + g < nc) ? S.dslot[4 * kk + g] : -1;
+            if (lb == 1) {
+                A4 d = A4{T(0), T(0), T(0), T(0)};
+#pragma unroll
+                for (int kk = 0; kk < 4; kk++) {
+                    const T la = S.linv[jj * L::LS + 4 * kk + g];
+                    const T bv = (dslc[kk] >= 0) ? S.sb[dslc[kk] * W + jj] : T(0);
+                    d = P::mfma(la, bv, d);
+                }
+#pragma unroll
+                for (int r = 0; r < 4; r++) S.u12[P::crow(g, r) * W + jj] = d[r];
+            }
+        } else if (lb != 0 && lfin) {
             const int l = lane_fresh(), g = l >> 4, jj = l & 15;
             T la[4];
             int dsl[4];
