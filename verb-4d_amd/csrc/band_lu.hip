@@ -259,7 +259,10 @@ struct Lds {
     static constexpr int PS = PR + 6;
     static constexpr int LS = NB + 2;        // row stride of linv (bank spread)
     T pnl[2][NB * PS];                       // panels p and p+1, column-major (full-row interchanges)
-    T u12[NB * W];
+    // U12 rows, stride WU = W + 1: the U-row stores read it a column at a
+    // time (16 lanes, 16 rows), which an even stride puts on one bank pair
+    static constexpr int WU = W + 1;
+    T u12[NB * WU];
     T sb[2 * NB * W];                        // pre-interchange contents of the rows that move
     T stash[(CB - 1) * 4 * 64];              // slots 1..CB-1 of the factoring wavefront's tiles
     T rw[PR];                                // right-hand-side window (index mod PR)
@@ -331,6 +334,7 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
     using P = Prec<T>;
     using A4 = typename P::acc4;
     constexpr int W = L::W;
+    constexpr int WU = L::WU;
     constexpr int PR = L::PR;
     constexpr int PS = L::PS;
     static_assert(RB >= 3, "the trailing update's U rows are written by row blocks 2..RB-1");
@@ -753,7 +757,7 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
                             d = P::mfma(la[kk], bv, d);
                         }
 #pragma unroll
-                        for (int r = 0; r < 4; r++) S.u12[P::crow(g, r) * W + NB * cb + jj] = d[r];
+                        for (int r = 0; r < 4; r++) S.u12[P::crow(g, r) * WU + NB * cb + jj] = d[r];
                     });
                     if (l == 0) __hip_atomic_fetch_add(&S.u12done, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
                 }
@@ -803,7 +807,7 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
                 A4 c4 = acc[s];
 #pragma unroll
                 for (int kk = 0; kk < 4; kk++) {
-                    const T bv = S.u12[(4 * kk + g) * W + NB * (s - 1) + jj];
+                    const T bv = S.u12[(4 * kk + g) * WU + NB * (s - 1) + jj];
                     c4 = P::mfma(an[kk], bv, c4);
                 }
                 acc[s] = c4;
@@ -848,7 +852,7 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
                         for (int r = 0; r < 4; r++) c4[r] = S.vx[(t * 4 + r) * 64 + l];
 #pragma unroll
                         for (int kk = 0; kk < 4; kk++) {
-                            const T bv = S.u12[(4 * kk + g) * W + NB * (1 + t) + jj];
+                            const T bv = S.u12[(4 * kk + g) * WU + NB * (1 + t) + jj];
                             c4 = P::mfma(ap[kk], bv, c4);
                         }
 #pragma unroll
@@ -929,7 +933,7 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
                     const int col_off = e >> 4;
                     const int col = j0 + col_off;
                     T v = T(0);
-                    if (col >= r && col <= ue) v = (col_off < NB) ? pn[col_off * PS + c] : S.u12[c * W + (col_off - NB)];
+                    if (col >= r && col <= ue) v = (col_off < NB) ? pn[col_off * PS + c] : S.u12[c * WU + (col_off - NB)];
                     dst[e] = v;
                 }
             } else if (lb >= 2) {
@@ -940,7 +944,7 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
                     const int col_off = e >> 4;
                     const int col = j0 + col_off;
                     if (col >= r && col <= ue) {
-                        const T v = (col_off < NB) ? pn[col_off * PS + c] : S.u12[c * W + (col_off - NB)];
+                        const T v = (col_off < NB) ? pn[col_off * PS + c] : S.u12[c * WU + (col_off - NB)];
                         ab[(size_t)col * ldab + kv + r - col] = v;
                     }
                 }
@@ -1116,7 +1120,7 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
                     d = P::mfma(la, bv, d);
                 }
 #pragma unroll
-                for (int r = 0; r < 4; r++) S.u12[P::crow(g, r) * W + jj] = d[r];
+                for (int r = 0; r < 4; r++) S.u12[P::crow(g, r) * WU + jj] = d[r];
             }
         } else if (lb != 0 && lfin) {
             const int l = lane_fresh(), g = l >> 4, jj = l & 15;
@@ -1152,7 +1156,7 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
                 const int cb = lb - 1 + t * (RB - 1);
                 if (cb < CB - 1)
 #pragma unroll
-                    for (int r = 0; r < 4; r++) S.u12[P::crow(g, r) * W + NB * cb + jj] = d[t][r];
+                    for (int r = 0; r < 4; r++) S.u12[P::crow(g, r) * WU + NB * cb + jj] = d[t][r];
             });
         } else if (lb != 0) {
             const int l = lane_fresh();
@@ -1176,7 +1180,7 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
                 });
                 static_for<0, NB>([&](auto cc) {
                     constexpr int c = cc;
-                    S.u12[c * W + t] = u[c];
+                    S.u12[c * WU + t] = u[c];
                 });
             }
         }
@@ -1223,7 +1227,7 @@ __global__ __launch_bounds__(RB * 64) __attribute__((amdgpu_waves_per_eu((sizeof
                     for (int kk = 0; kk < 4; kk++) an[kk] = -pn[(4 * kk + g) * PS + NB * lb + jj];
                     A4 c4 = acc[1];
 #pragma unroll
-                    for (int kk = 0; kk < 4; kk++) c4 = P::mfma(an[kk], S.u12[(4 * kk + g) * W + jj], c4);
+                    for (int kk = 0; kk < 4; kk++) c4 = P::mfma(an[kk], S.u12[(4 * kk + g) * WU + jj], c4);
                     acc[1] = c4;
                 }
 #pragma unroll
